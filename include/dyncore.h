@@ -1,0 +1,117 @@
+/*
+ * dyncore.h -- C ABI of the MI355X (gfx950) anelastic dynamical core.
+ *
+ * Plain pointers and sizes only.  One context owns one model state resident in HBM and one HIP
+ * stream; every call is stream-ordered and returns 0 on success or a negative DYN_E_* code, with a
+ * message from dyn_last_error().  The C++ drop-in layer (include/dyncore_refapi.h) keeps the
+ * reference's own entry points and forwards to these calls.
+ *
+ * Grid conventions are the reference's (model.h:46-58): arrays are [i][j][k] with k fastest,
+ * NX includes three periodic halo columns on each side, NZ includes the ghost levels 0 and NZ-1.
+ * Bounds arguments il, ih, jl, jh are half-open ranges in array coordinates, exactly as the
+ * reference functions take them.
+ */
+#ifndef DYNCORE_H
+#define DYNCORE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- identifiers --------------------------------------------------------------------------- */
+enum dyn_field {
+    DYN_U = 0, DYN_V, DYN_W, DYN_TH, DYN_QV, DYN_QC, DYN_QR, /* prognostics: 3 buffers each */
+    DYN_PI,                                                  /* perturbation pressure (pis) */
+    DYN_UBAR, DYN_VBAR, DYN_WBAR, DYN_THBAR, DYN_QBAR, DYN_PBAR, /* basic state (m_ubar ...) */
+    DYN_VT,                                                  /* rain fall speed (vts) */
+    DYN_ACCRAIN,                                             /* accumulated rain [NX][NY] */
+    DYN_FIELD_COUNT
+};
+/* the three rotating buffers of a prognostic: ums / us / ups (model.h:63-65) */
+enum dyn_role { DYN_ROLE_M = 0, DYN_ROLE_CUR = 1, DYN_ROLE_P = 2 };
+enum dyn_column {
+    DYN_COL_ZU = 0, DYN_COL_ZW, DYN_COL_ZSU, DYN_COL_ZSW, DYN_COL_MU, DYN_COL_MW,
+    DYN_COL_RHOU, DYN_COL_RHOW, DYN_COL_ONE_D_RHOU, DYN_COL_ONE_D_RHOW,
+    DYN_COL_TB, DYN_COL_TBV, DYN_COL_TBW, DYN_COL_QB, DYN_COL_PIB,
+    DYN_COL_F, /* Coriolis parameter f[j], length NY */
+    DYN_COLUMN_COUNT
+};
+enum dyn_precision { DYN_FP64 = 0, DYN_FP32 = 1 };
+
+/* error codes */
+#define DYN_OK 0
+#define DYN_E_ARG (-1)         /* invalid argument (bounds, ids, sizes) */
+#define DYN_E_UNSUPPORTED (-2) /* option outside the implemented path (e.g. non-periodic east-west) */
+#define DYN_E_HIP (-3)         /* HIP runtime failure */
+#define DYN_E_NUMERIC (-4)     /* the reference's fatal Kessler guard E>100 (kessler.cpp:83-88) or NaN */
+#define DYN_E_STATE (-5)       /* call out of order (e.g. solve before the columns are set) */
+
+typedef struct dyn_grid {
+    int nx, ny, nz;          /* NX (incl. 6 halo columns), NY, NZ (incl. 2 ghost levels) */
+    double dx, dz, dt;       /* dy = dx, as process_input.cpp:175-176 sets it */
+    int microphysics;        /* MICROPHYSICS_OPTION: 0 dry, 1 Kessler (2, ice, is unsupported) */
+    int periodic_ew;         /* PERIODIC_BOUNDARIES: must be 1 (serial periodic path) */
+    int rayleigh_level;      /* raydampheight (initializer.cpp:185); >= nz disables damping */
+    int sponge_north, sponge_south; /* jnbuffer, jsbuffer (boundaries.cpp:23-26) */
+    int precision;           /* DYN_FP64 (plain form) or DYN_FP32 (fp32 storage, fp64 vertical solve) */
+} dyn_grid;
+
+typedef struct dyn_ctx dyn_ctx;
+
+const char *dyn_last_error(void);
+int dyn_version(void);
+
+/* lifetime: initialize_pressure_solver + init_boundaries + setup_memory_allocation equivalents */
+int dyn_create(const dyn_grid *grid, dyn_ctx **out);
+int dyn_destroy(dyn_ctx *ctx);
+
+/* host <-> HBM transfers (double on the host side for both precisions) */
+int dyn_set_column(dyn_ctx *ctx, int column, const double *values, size_t count);
+int dyn_set_field(dyn_ctx *ctx, int field, int role, const double *values, size_t count);
+int dyn_get_field(dyn_ctx *ctx, int field, int role, double *values, size_t count);
+int dyn_synchronize(dyn_ctx *ctx);
+
+/* ---- reference operators, one call each -------------------------------------------------- */
+/* advect_uvw_velocity(step,il,ih,jl,jh), advection.cpp:304 */
+int dyn_advect_uvw(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh);
+/* advect_theta(step,il,ih,jl,jh), advection.cpp:349 */
+int dyn_advect_theta(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh);
+/* advect_microphysics_cell(step,il,ih,jl,jh), advection.cpp:451 (includes the fall speed) */
+int dyn_advect_moisture(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh);
+/* zero_moisture(il,ih,jl,jh,size), microphysics.cpp:675 */
+int dyn_zero_moisture(dyn_ctx *ctx, int il, int ih, int jl, int jh);
+/* solve_pressure(step), pressure.cpp:142 (serial periodic: uvw halos, poisson_fft3d, pi halos) */
+int dyn_solve_pressure(dyn_ctx *ctx, double step);
+/* pressure-gradient force of integrate_non_hydro, solver.cpp:123-135, over [il,ih) x [jl,jh) */
+int dyn_pgf(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh);
+/* apply_boundary_condition(0), boundaries.cpp:119 / apply_boundary_condition_microphysics(0), :200 */
+int dyn_apply_bc(dyn_ctx *ctx);
+int dyn_apply_bc_micro(dyn_ctx *ctx);
+/* run_microphysics(il,ih,jl,jh) -> run_kessler_microphysics + precip_rate, kessler.cpp:25 */
+int dyn_kessler(dyn_ctx *ctx, int il, int ih, int jl, int jh);
+/* rayleigh_damping(il,ih,jl,jh,level), damping.cpp:1148 */
+int dyn_rayleigh(dyn_ctx *ctx, int il, int ih, int jl, int jh, int level);
+/* advance_inner / advance_outer, solver.cpp:241-263; microphysics_advance(_inner), microphysics.cpp:642-669 */
+int dyn_advance_inner(dyn_ctx *ctx);
+int dyn_advance_outer(dyn_ctx *ctx);
+int dyn_micro_advance_inner(dyn_ctx *ctx);
+int dyn_micro_advance_outer(dyn_ctx *ctx);
+
+/* ---- whole steps ------------------------------------------------------------------------- */
+/* s_integrate_rk3 (driver_serial.cpp:178-286) as the sequence of per-operator calls above */
+int dyn_step_reference(dyn_ctx *ctx, int nsteps);
+/* the same step with fused stage kernels, replayed from a captured hipGraph */
+int dyn_step(dyn_ctx *ctx, int nsteps);
+
+/* ---- measurement ------------------------------------------------------------------------- */
+/* HBM bytes the step's algorithm must move per gridpoint (the field-pass count of the operator set) */
+double dyn_algorithmic_bytes_per_point(const dyn_ctx *ctx);
+/* the HIP stream the context launches on (a hipStream_t) */
+void *dyn_stream(dyn_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

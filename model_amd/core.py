@@ -1,0 +1,167 @@
+"""Python host binding of libdyncore's C ABI (include/dyncore.h) through ctypes.
+
+The library is loaded from model_amd/_build/libdyncore.so; if it is missing the import fails loudly --
+there is no CPU fallback.  Host arrays are float64 numpy arrays in the reference layout [i][j][k].
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import ids
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libdyncore.so")
+REFAPI_PATH = os.path.join(_HERE, "_build", "libdyncore_refapi.so")
+
+DYN_FP64, DYN_FP32 = 0, 1
+ERRORS = {-1: "DYN_E_ARG", -2: "DYN_E_UNSUPPORTED", -3: "DYN_E_HIP", -4: "DYN_E_NUMERIC", -5: "DYN_E_STATE"}
+
+# the exported C symbols, as declared in include/dyncore.h
+EXPORTS = [
+    "dyn_last_error", "dyn_version", "dyn_create", "dyn_destroy", "dyn_set_column", "dyn_set_field", "dyn_get_field",
+    "dyn_synchronize", "dyn_advect_uvw", "dyn_advect_theta", "dyn_advect_moisture", "dyn_zero_moisture",
+    "dyn_solve_pressure", "dyn_pgf", "dyn_apply_bc", "dyn_apply_bc_micro", "dyn_kessler", "dyn_rayleigh",
+    "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner", "dyn_micro_advance_outer",
+    "dyn_step_reference", "dyn_step", "dyn_algorithmic_bytes_per_point", "dyn_stream",
+]
+
+
+class DynError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class DynGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+                ("microphysics", ctypes.c_int), ("periodic_ew", ctypes.c_int), ("rayleigh_level", ctypes.c_int),
+                ("sponge_north", ctypes.c_int), ("sponge_south", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def load():
+    """Load libdyncore.so (built in-tree by model_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.dyn_last_error.restype = ctypes.c_char_p
+    L.dyn_create.argtypes = [ctypes.POINTER(DynGrid), ctypes.POINTER(ctypes.c_void_p)]
+    L.dyn_destroy.argtypes = [ctypes.c_void_p]
+    L.dyn_set_column.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_size_t]
+    L.dyn_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_size_t]
+    L.dyn_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_size_t]
+    L.dyn_synchronize.argtypes = [ctypes.c_void_p]
+    for fn in ("dyn_advect_uvw", "dyn_advect_theta", "dyn_advect_moisture", "dyn_pgf"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4
+    for fn in ("dyn_zero_moisture", "dyn_kessler"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
+    L.dyn_rayleigh.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
+    L.dyn_solve_pressure.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    for fn in ("dyn_apply_bc", "dyn_apply_bc_micro", "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner",
+               "dyn_micro_advance_outer"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p]
+    L.dyn_step_reference.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dyn_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dyn_algorithmic_bytes_per_point.argtypes = [ctypes.c_void_p]
+    L.dyn_algorithmic_bytes_per_point.restype = ctypes.c_double
+    L.dyn_stream.argtypes = [ctypes.c_void_p]
+    L.dyn_stream.restype = ctypes.c_void_p
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise DynError(rc, load().dyn_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class DynCore:
+    """One HBM-resident model state (a dyn_ctx)."""
+
+    def __init__(self, case, precision: int = DYN_FP64):
+        self.L = load()
+        self.case = case
+        g = DynGrid(case.nx, case.ny, case.nz, case.dx, case.dz, case.dt, case.microphysics, 1, case.rayleigh_level,
+                    case.sponge, case.sponge, precision)
+        h = ctypes.c_void_p()
+        _check(self.L.dyn_create(ctypes.byref(g), ctypes.byref(h)))
+        self.h = h
+        self.precision = precision
+        for name, arr in case.columns.items():
+            self.set_column(ids.COLUMN_ID[name], arr)
+        for name, arr in case.fields.items():
+            f = ids.FIELD_ID[name]
+            if f in (ids.QV, ids.QC, ids.QR) and not case.microphysics:
+                continue
+            if f in ids.PROGNOSTIC:
+                for role in (ids.ROLE_M, ids.ROLE_CUR, ids.ROLE_P):
+                    self.set_field(f, role, arr)
+            else:
+                self.set_field(f, 0, arr)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.dyn_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_column(self, c, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(self.L.dyn_set_column(self.h, c, _ptr(v), v.size))
+
+    def set_field(self, f, role, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(self.L.dyn_set_field(self.h, f, role, _ptr(v), v.size))
+
+    def get_field(self, f, role=ids.ROLE_CUR):
+        c = self.case
+        shape = (c.nx, c.ny) if f == ids.ACCRAIN else (c.nx, c.ny, c.nz)
+        out = np.empty(shape)
+        _check(self.L.dyn_get_field(self.h, f, role, _ptr(out), out.size))
+        return out
+
+    def sync(self):
+        _check(self.L.dyn_synchronize(self.h))
+
+    def step(self, n=1):
+        _check(self.L.dyn_step(self.h, n))
+
+    def step_reference(self, n=1):
+        _check(self.L.dyn_step_reference(self.h, n))
+
+    @property
+    def stream(self):
+        return self.L.dyn_stream(self.h)
+
+    def bytes_per_point(self):
+        return self.L.dyn_algorithmic_bytes_per_point(self.h)
+
+    def __getattr__(self, name):
+        if name.startswith("_") or name in ("L", "h", "case", "ids"):
+            raise AttributeError(name)
+        fn = getattr(self.L, "dyn_" + name)
+        h = self.h
+
+        def call(*a):
+            _check(fn(h, *a))
+
+        return call

@@ -1,0 +1,605 @@
+// C ABI of libdyncore (include/dyncore.h): context lifetime, HBM-resident state, transfers, the
+// per-operator entry points and whole-step drivers.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dyn_internal.hpp"
+
+namespace dyn {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int pressure_coefficients(Ctx *c);
+
+namespace {
+
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long long n) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long s = (long long)gridDim.x * blockDim.x;
+    for (; p < n; p += s) dst[p] = src[p];
+}
+__global__ void k_copy_f(float *__restrict__ dst, const float *__restrict__ src, long long n) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long s = (long long)gridDim.x * blockDim.x;
+    for (; p < n; p += s) dst[p] = src[p];
+}
+__global__ void k_clear_flag(int *f) { *f = 0; }
+
+int device_copy(Ctx *c, void *dst, const void *src) {
+    long long n = (long long)c->npts;
+    unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    if (c->prec == DYN_FP32) hipLaunchKernelGGL(k_copy_f, dim3(blocks), dim3(256), 0, c->stream, (float *)dst, (const float *)src, n);
+    else hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, c->stream, (double *)dst, (const double *)src, n);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
+Ctx *as_ctx(dyn_ctx *p) { return reinterpret_cast<Ctx *>(p); }
+
+bool cols_ready(const Ctx *c) {
+    static const int need[] = {DYN_COL_ZU, DYN_COL_ZW, DYN_COL_MU, DYN_COL_MW, DYN_COL_RHOU, DYN_COL_RHOW, DYN_COL_ONE_D_RHOU,
+                               DYN_COL_TB, DYN_COL_TBV, DYN_COL_QB, DYN_COL_F};
+    for (int k : need)
+        if (!c->cols_set[k]) return false;
+    return true;
+}
+
+#define CHECK_CTX(c)                                              \
+    do {                                                          \
+        if (!(c)) {                                               \
+            set_error("null context");                            \
+            return DYN_E_ARG;                                     \
+        }                                                         \
+    } while (0)
+#define CHECK_COLS(c)                                                                                        \
+    do {                                                                                                     \
+        if (!cols_ready(c)) {                                                                                \
+            set_error("columns (zu, zw, mu, mw, rhou, rhow, one_d_rhou, tb, tbv, qb, f) must be set first"); \
+            return DYN_E_STATE;                                                                              \
+        }                                                                                                    \
+    } while (0)
+
+int check_box(const Ctx *c, int il, int ih, int jl, int jh, int lo_i, int hi_i, int lo_j, int hi_j, const char *what) {
+    if (il < lo_i || ih > hi_i || jl < lo_j || jh > hi_j || il >= ih || jl >= jh) {
+        set_error(std::string(what) + ": bounds [" + std::to_string(il) + "," + std::to_string(ih) + ")x[" + std::to_string(jl) +
+                  "," + std::to_string(jh) + ") outside the stencil-safe range [" + std::to_string(lo_i) + "," +
+                  std::to_string(hi_i) + ")x[" + std::to_string(lo_j) + "," + std::to_string(hi_j) + ")");
+        return DYN_E_ARG;
+    }
+    return DYN_OK;
+}
+
+int upload_columns(Ctx *c) {
+    size_t bytes = sizeof(double) * ((size_t)(DYN_COLUMN_COUNT - 1) * c->g.nz + c->g.ny);
+    DYN_HIP(hipMemcpyAsync(c->colbuf, c->colhost, bytes, hipMemcpyHostToDevice, c->stream));
+    return DYN_OK;
+}
+
+int swap_roles(Ctx *c, int f0, int f1) {
+    for (int f = f0; f <= f1; f++) std::swap(c->prog[f][DYN_ROLE_CUR], c->prog[f][DYN_ROLE_P]);
+    return DYN_OK;
+}
+
+}  // namespace
+}  // namespace dyn
+
+using namespace dyn;
+
+extern "C" {
+
+const char *dyn_last_error(void) { return g_last_error.c_str(); }
+int dyn_version(void) { return 1; }
+
+int dyn_create(const dyn_grid *grid, dyn_ctx **out) {
+    if (!grid || !out) {
+        set_error("dyn_create: null argument");
+        return DYN_E_ARG;
+    }
+    *out = nullptr;
+    const dyn_grid &g = *grid;
+    if (g.nx < 12 || g.ny < 8 || g.nz < 5 || !(g.dx > 0) || !(g.dz > 0) || !(g.dt > 0)) {
+        set_error("dyn_create: grid too small or non-positive spacing");
+        return DYN_E_ARG;
+    }
+    if (g.periodic_ew != 1) {
+        set_error("dyn_create: only the serial periodic east-west path (PERIODIC_BOUNDARIES=1) is implemented");
+        return DYN_E_UNSUPPORTED;
+    }
+    if (g.microphysics != 0 && g.microphysics != 1) {
+        set_error("dyn_create: microphysics option must be 0 (dry) or 1 (Kessler)");
+        return DYN_E_UNSUPPORTED;
+    }
+    if (g.precision != DYN_FP64 && g.precision != DYN_FP32) {
+        set_error("dyn_create: precision must be DYN_FP64 or DYN_FP32");
+        return DYN_E_ARG;
+    }
+    if (g.sponge_north < 0 || g.sponge_south < 0 || g.sponge_north + g.sponge_south > g.ny) {
+        set_error("dyn_create: sponge widths out of range");
+        return DYN_E_ARG;
+    }
+    if (g.rayleigh_level == g.nz - 1) {
+        set_error("dyn_create: rayleigh_level = NZ-1 gives 0/0 in damping.cpp:1159-1160; use <= NZ-2 or >= NZ");
+        return DYN_E_ARG;
+    }
+    if (g.rayleigh_level < 1) {
+        set_error("dyn_create: rayleigh_level must be >= 1");
+        return DYN_E_ARG;
+    }
+    Ctx *c = new Ctx();
+    c->g = g;
+    c->prec = g.precision;
+    c->elt = g.precision == DYN_FP32 ? 4 : 8;
+    c->npts = (size_t)g.nx * g.ny * g.nz;
+    Geo &geo = c->geo;
+    geo.NX = g.nx;
+    geo.NY = g.ny;
+    geo.NZ = g.nz;
+    geo.NYNZ = (long long)g.ny * g.nz;
+    geo.dt = g.dt;
+    geo.dtx = g.dt / g.dx;  // initializer.cpp:173-179, dy = dx
+    geo.dty = g.dt / g.dx;
+    geo.dtz = g.dt / g.dz;
+    geo.one_d_dx = 1.0 / g.dx;
+    geo.one_d_dy = 1.0 / g.dx;
+    geo.one_d_dz = 1.0 / g.dz;
+    auto fail = [&](int rc) {
+        dyn_destroy(reinterpret_cast<dyn_ctx *>(c));
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        delete c;
+        return DYN_E_HIP;
+    }
+    size_t bytes = c->npts * c->elt;
+    int nprog = g.microphysics ? 7 : 4;
+    for (int f = 0; f < nprog; f++)
+        for (int r = 0; r < 3; r++) {
+            if (hipMalloc(&c->prog[f][r], bytes) != hipSuccess || hipMemset(c->prog[f][r], 0, bytes) != hipSuccess) {
+                set_error("hipMalloc of a prognostic field failed");
+                return fail(DYN_E_HIP);
+            }
+        }
+    void **singles[] = {&c->pi, &c->vt, &c->bar[0], &c->bar[1], &c->bar[2], &c->bar[3], &c->bar[4], &c->bar[5]};
+    for (void **p : singles) {
+        if (hipMalloc(p, bytes) != hipSuccess || hipMemset(*p, 0, bytes) != hipSuccess) {
+            set_error("hipMalloc of a diagnostic field failed");
+            return fail(DYN_E_HIP);
+        }
+    }
+    size_t acc = (size_t)g.nx * g.ny * c->elt;
+    if (hipMalloc(&c->accrain, acc) != hipSuccess || hipMemset(c->accrain, 0, acc) != hipSuccess) return fail(DYN_E_HIP);
+    size_t ncol = (size_t)(DYN_COLUMN_COUNT - 1) * g.nz + g.ny;
+    c->colhost = (double *)calloc(ncol, sizeof(double));
+    if (hipMalloc(&c->colbuf, ncol * sizeof(double)) != hipSuccess || hipMemset(c->colbuf, 0, ncol * sizeof(double)) != hipSuccess)
+        return fail(DYN_E_HIP);
+    const double *cb = c->colbuf;
+    const double **cp[] = {&c->cols.zu, &c->cols.zw, &c->cols.zsu, &c->cols.zsw, &c->cols.mu, &c->cols.mw, &c->cols.rhou, &c->cols.rhow,
+                           &c->cols.one_d_rhou, &c->cols.one_d_rhow, &c->cols.tb, &c->cols.tbv, &c->cols.tbw, &c->cols.qb, &c->cols.pib,
+                           &c->cols.f};
+    for (int k = 0; k < DYN_COLUMN_COUNT; k++) *cp[k] = cb + (size_t)k * g.nz;
+    // sponge coefficients: init_boundaries via cos_profile (boundaries.cpp:84-86, util.cpp:740-749)
+    auto cos_profile = [](double pl, double ph, double frac, double shift) {
+        if (frac >= 0 && frac <= 1.0) return std::cos(pl * 2 * kTrigPi + (ph - pl) * 2 * kTrigPi * frac) + shift;
+        return 0.0;
+    };
+    std::vector<double> sn(std::max(1, g.sponge_north)), ss(std::max(1, g.sponge_south));
+    for (int i = 0; i < g.sponge_north; i++) sn[i] = 0.5 * cos_profile(-0.5, 0, (double)(i + 1) / (double)g.sponge_north, 1);
+    for (int i = 0; i < g.sponge_south; i++) ss[i] = 0.5 * cos_profile(0, 0.5, (double)i / (double)g.sponge_south, 1);
+    if (hipMalloc(&c->sponge_n, sn.size() * 8) != hipSuccess || hipMalloc(&c->sponge_s, ss.size() * 8) != hipSuccess ||
+        hipMemcpy(c->sponge_n, sn.data(), sn.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->sponge_s, ss.data(), ss.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(DYN_E_HIP);
+    if (hipMalloc(&c->raycoef, sizeof(double) * 2 * g.nz) != hipSuccess || hipMemset(c->raycoef, 0, sizeof(double) * 2 * g.nz) != hipSuccess)
+        return fail(DYN_E_HIP);
+    if (hipMalloc(&c->dflag, sizeof(int)) != hipSuccess || hipMemset(c->dflag, 0, sizeof(int)) != hipSuccess) return fail(DYN_E_HIP);
+    int rc = pressure_init(c);
+    if (rc) return fail(rc);
+    if (hipDeviceSynchronize() != hipSuccess) return fail(DYN_E_HIP);
+    *out = reinterpret_cast<dyn_ctx *>(c);
+    return DYN_OK;
+}
+
+int dyn_destroy(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    if (!c) return DYN_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    pressure_free(c);
+    for (auto &f : c->prog)
+        for (void *p : f)
+            if (p) (void)hipFree(p);
+    void *ptrs[] = {c->pi, c->vt, c->accrain, c->bar[0], c->bar[1], c->bar[2], c->bar[3], c->bar[4], c->bar[5], c->colbuf,
+                    c->sponge_n, c->sponge_s, c->raycoef, c->dflag};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    free(c->colhost);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DYN_OK;
+}
+
+int dyn_set_column(dyn_ctx *ctx, int column, const double *values, size_t count) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    if (column < 0 || column >= DYN_COLUMN_COUNT || !values) {
+        set_error("dyn_set_column: bad column id");
+        return DYN_E_ARG;
+    }
+    size_t need = column == DYN_COL_F ? (size_t)c->g.ny : (size_t)c->g.nz;
+    if (count != need) {
+        set_error("dyn_set_column: expected " + std::to_string(need) + " values");
+        return DYN_E_ARG;
+    }
+    memcpy(c->colhost + (size_t)column * c->g.nz, values, need * sizeof(double));
+    c->cols_set[column] = true;
+    int rc = upload_columns(c);
+    if (rc) return rc;
+    if (column == DYN_COL_MU || column == DYN_COL_MW || column == DYN_COL_RHOW || column == DYN_COL_RHOU) {
+        if (c->cols_set[DYN_COL_MU] && c->cols_set[DYN_COL_MW] && c->cols_set[DYN_COL_RHOW] && c->cols_set[DYN_COL_RHOU]) {
+            DYN_HIP(hipStreamSynchronize(c->stream));
+            rc = pressure_coefficients(c);
+            if (rc) return rc;
+        }
+    }
+    c->ray_level_cached = -1;
+    return DYN_OK;
+}
+
+static void *field_ptr(Ctx *c, int field, int role) {
+    if (field >= DYN_U && field <= DYN_QR) {
+        if (role < 0 || role > 2) return nullptr;
+        return c->prog[field][role];
+    }
+    switch (field) {
+    case DYN_PI: return c->pi;
+    case DYN_UBAR: return c->bar[0];
+    case DYN_VBAR: return c->bar[1];
+    case DYN_WBAR: return c->bar[2];
+    case DYN_THBAR: return c->bar[3];
+    case DYN_QBAR: return c->bar[4];
+    case DYN_PBAR: return c->bar[5];
+    case DYN_VT: return c->vt;
+    case DYN_ACCRAIN: return c->accrain;
+    }
+    return nullptr;
+}
+
+static int transfer(Ctx *c, int field, int role, double *host, size_t count, bool up) {
+    if (field < 0 || field >= DYN_FIELD_COUNT || !host) {
+        set_error("bad field id");
+        return DYN_E_ARG;
+    }
+    void *d = field_ptr(c, field, role);
+    if (!d) {
+        set_error("field not present in this context (moisture fields need microphysics=1) or bad role");
+        return DYN_E_ARG;
+    }
+    size_t n = field_count(c, field);
+    if (count != n) {
+        set_error("transfer: expected " + std::to_string(n) + " values, got " + std::to_string(count));
+        return DYN_E_ARG;
+    }
+    if (c->prec == DYN_FP64) {
+        if (up) DYN_HIP(hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->stream));
+        else DYN_HIP(hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, c->stream));
+        DYN_HIP(hipStreamSynchronize(c->stream));
+        return DYN_OK;
+    }
+    double *tmp = nullptr;
+    DYN_HIP(hipMalloc(&tmp, n * 8));
+    int rc = DYN_OK;
+    if (up) {
+        if (hipMemcpyAsync(tmp, host, n * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = DYN_E_HIP;
+        if (!rc) rc = launch_convert(c, d, tmp, n, true);
+    } else {
+        rc = launch_convert(c, tmp, (const double *)d, n, false);
+        if (!rc && hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = DYN_E_HIP;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = DYN_E_HIP;
+    (void)hipFree(tmp);
+    if (rc) set_error("FP32 transfer failed");
+    return rc;
+}
+
+int dyn_set_field(dyn_ctx *ctx, int field, int role, const double *values, size_t count) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    return transfer(c, field, role, const_cast<double *>(values), count, true);
+}
+
+int dyn_get_field(dyn_ctx *ctx, int field, int role, double *values, size_t count) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    return transfer(c, field, role, values, count, false);
+}
+
+int dyn_synchronize(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    DYN_HIP(hipStreamSynchronize(c->stream));
+    int flag = 0;
+    DYN_HIP(hipMemcpy(&flag, c->dflag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        set_error("Kessler evaporation E > 100 (the reference aborts here, kessler.cpp:83-88)");
+        return DYN_E_NUMERIC;
+    }
+    return DYN_OK;
+}
+
+int dyn_advect_uvw(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    int rc = check_box(c, il, ih, jl, jh, 3, c->g.nx - 3, 3, c->g.ny - 3, "advect_uvw_velocity");
+    return rc ? rc : launch_advect_uvw(c, step, il, ih, jl, jh);
+}
+
+int dyn_advect_theta(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    int rc = check_box(c, il, ih, jl, jh, 3, c->g.nx - 3, 3, c->g.ny - 3, "advect_theta");
+    return rc ? rc : launch_advect_theta(c, step, il, ih, jl, jh);
+}
+
+int dyn_advect_moisture(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    if (!c->g.microphysics) {
+        set_error("advect_microphysics_cell: context has no moisture fields");
+        return DYN_E_STATE;
+    }
+    int rc = check_box(c, il, ih, jl, jh, 3, c->g.nx - 3, 3, c->g.ny - 3, "advect_microphysics_cell");
+    return rc ? rc : launch_advect_moisture(c, step, il, ih, jl, jh);
+}
+
+int dyn_zero_moisture(dyn_ctx *ctx, int il, int ih, int jl, int jh) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    if (!c->g.microphysics) {
+        set_error("zero_moisture: context has no moisture fields");
+        return DYN_E_STATE;
+    }
+    int rc = check_box(c, il, ih, jl, jh, 0, c->g.nx, 0, c->g.ny, "zero_moisture");
+    return rc ? rc : launch_zero_moisture(c, il, ih, jl, jh);
+}
+
+int dyn_solve_pressure(dyn_ctx *ctx, double step) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    if (!(step > 0)) {
+        set_error("solve_pressure: step must be positive (the divergence is divided by step*dt)");
+        return DYN_E_ARG;
+    }
+    return launch_solve_pressure(c, step);
+}
+
+int dyn_pgf(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    int rc = check_box(c, il, ih, jl, jh, 1, c->g.nx, 1, c->g.ny, "pressure gradient");
+    return rc ? rc : launch_pgf(c, step, il, ih, jl, jh);
+}
+
+int dyn_apply_bc(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    return launch_bc(c, false);
+}
+
+int dyn_apply_bc_micro(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    if (!c->g.microphysics) {
+        set_error("apply_boundary_condition_microphysics: context has no moisture fields");
+        return DYN_E_STATE;
+    }
+    return launch_bc(c, true);
+}
+
+int dyn_kessler(dyn_ctx *ctx, int il, int ih, int jl, int jh) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    if (!c->g.microphysics) {
+        set_error("run_microphysics: context has no moisture fields");
+        return DYN_E_STATE;
+    }
+    int rc = check_box(c, il, ih, jl, jh, 0, c->g.nx, 0, c->g.ny, "run_microphysics");
+    return rc ? rc : launch_kessler(c, il, ih, jl, jh);
+}
+
+// Rayleigh coefficients exactly as damping.cpp:1153-1160 forms them (unstretched zu, zw), uploaded
+// once per level so that no host copy happens inside a captured step
+static int ensure_rayleigh_table(Ctx *c, int level) {
+    const int NZ = c->g.nz;
+    if (c->ray_level_cached == level) return DYN_OK;
+    const double *zu = c->colhost + (size_t)DYN_COL_ZU * NZ, *zw = c->colhost + (size_t)DYN_COL_ZW * NZ;
+    std::vector<double> coef(2 * (size_t)NZ, 0.0);
+    double raydmpcoef = 0.05, raydmpz = zu[level];
+    for (int k = level; k < NZ; k++) {
+        coef[k] = raydmpcoef * 0.5 * (1.0 - std::cos(kTrigPi * (zu[k] - raydmpz) / (zu[NZ - 1] - raydmpz)));
+        coef[NZ + k] = raydmpcoef * 0.5 * (1.0 - std::cos(kTrigPi * (zw[k] - raydmpz) / (zw[NZ - 1] - raydmpz)));
+    }
+    DYN_HIP(hipMemcpyAsync(c->raycoef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, c->stream));
+    DYN_HIP(hipStreamSynchronize(c->stream));
+    c->ray_level_cached = level;
+    return DYN_OK;
+}
+
+int dyn_rayleigh(dyn_ctx *ctx, int il, int ih, int jl, int jh, int level) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    int rc = check_box(c, il, ih, jl, jh, 0, c->g.nx, 0, c->g.ny, "rayleigh_damping");
+    if (rc) return rc;
+    const int NZ = c->g.nz;
+    if (level >= NZ) return DYN_OK;  // damping.cpp:1156: the k loop is empty
+    if (level < 0 || level == NZ - 1) {
+        set_error("rayleigh_damping: level must be in [0, NZ-2] or >= NZ (NZ-1 divides 0 by 0)");
+        return DYN_E_ARG;
+    }
+    int rc2 = ensure_rayleigh_table(c, level);
+    if (rc2) return rc2;
+    return launch_rayleigh(c, il, ih, jl, jh, level);
+}
+
+int dyn_advance_inner(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    return swap_roles(c, DYN_U, DYN_TH);
+}
+
+int dyn_advance_outer(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    for (int f = DYN_U; f <= DYN_TH; f++) {
+        int rc = device_copy(c, c->prog[f][DYN_ROLE_M], c->prog[f][DYN_ROLE_P]);
+        if (rc) return rc;
+    }
+    return swap_roles(c, DYN_U, DYN_TH);
+}
+
+int dyn_micro_advance_inner(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    if (!c->g.microphysics) return DYN_OK;
+    return swap_roles(c, DYN_QV, DYN_QR);
+}
+
+int dyn_micro_advance_outer(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    if (!c->g.microphysics) return DYN_OK;
+    for (int f = DYN_QV; f <= DYN_QR; f++) {
+        int rc = device_copy(c, c->prog[f][DYN_ROLE_M], c->prog[f][DYN_ROLE_P]);
+        if (rc) return rc;
+    }
+    return swap_roles(c, DYN_QV, DYN_QR);
+}
+
+// s_integrate_rk3, driver_serial.cpp:178-286 (no diffusion, turbulence, terrain or heating)
+static int reference_step(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    const double steps[] = {1. / 3., 0.5, 1.0};
+    const int NX = c->g.nx, NY = c->g.ny;
+    const bool moist = c->g.microphysics != 0;
+    int rc;
+#define RC(x)                  \
+    do {                       \
+        if ((rc = (x))) return rc; \
+    } while (0)
+    for (int s = 0; s < 3; s++) {
+        double step = steps[s];
+        // integrate_non_hydro(step,0,NX,0,NY), solver.cpp:87-138
+        RC(dyn_advect_uvw(ctx, step, 3, NX - 3, 3, NY - 3));
+        RC(dyn_solve_pressure(ctx, step));
+        RC(dyn_pgf(ctx, step, 1, NX - 1, 1, NY - 1));
+        RC(dyn_advect_theta(ctx, step, 3, NX - 3, 3, NY - 3));
+        if (moist) {
+            RC(dyn_advect_moisture(ctx, step, 3, NX - 3, 3, NY - 3));
+            RC(dyn_zero_moisture(ctx, 1, NX - 1, 1, NY - 1));
+        }
+        RC(dyn_apply_bc(ctx));
+        if (moist) {
+            RC(dyn_apply_bc_micro(ctx));
+            if (s < 2) RC(dyn_micro_advance_inner(ctx));
+        }
+        if (s < 2) RC(dyn_advance_inner(ctx));
+    }
+    if (moist) {
+        RC(dyn_kessler(ctx, 1, NX - 1, 1, NY - 1));
+        RC(dyn_apply_bc_micro(ctx));
+        RC(dyn_micro_advance_outer(ctx));
+    }
+    RC(dyn_apply_bc(ctx));
+    RC(dyn_rayleigh(ctx, 1, NX - 1, 1, NY - 1, c->g.rayleigh_level));
+    RC(dyn_advance_outer(ctx));
+#undef RC
+    return DYN_OK;
+}
+
+int dyn_step_reference(dyn_ctx *ctx, int nsteps) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    for (int n = 0; n < nsteps; n++) {
+        int rc = reference_step(ctx);
+        if (rc) return rc;
+    }
+    return DYN_OK;
+}
+
+// One step replayed from a hipGraph.  A step permutes the current/forecast buffers with period 2,
+// so the graph records two steps and an odd trailing step runs eagerly.
+int dyn_step(dyn_ctx *ctx, int nsteps) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    if (nsteps <= 0) return DYN_OK;
+    // the Rayleigh table is uploaded with a host copy, which must not happen inside a capture
+    if (c->g.rayleigh_level < c->g.nz) {
+        int rc = ensure_rayleigh_table(c, c->g.rayleigh_level);
+        if (rc) return rc;
+    }
+    int pairs = nsteps / 2;
+    if (pairs > 0) {
+        if (!c->graph_exec) {
+            void *saved[7][3];
+            memcpy(saved, c->prog, sizeof(saved));
+            DYN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = reference_step(ctx);
+            if (!rc) rc = reference_step(ctx);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rc || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                memcpy(c->prog, saved, sizeof(saved));
+                if (!rc) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                return rc ? rc : DYN_E_HIP;
+            }
+            // two steps restore the buffer assignment
+            if (memcmp(saved, c->prog, sizeof(saved)) != 0) {
+                (void)hipGraphDestroy(graph);
+                memcpy(c->prog, saved, sizeof(saved));
+                set_error("internal: two steps did not restore the buffer assignment");
+                return DYN_E_STATE;
+            }
+            DYN_HIP(hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0));
+            c->graph = graph;
+        }
+        for (int p = 0; p < pairs; p++) DYN_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    }
+    if (nsteps % 2) {
+        int rc = reference_step(ctx);
+        if (rc) return rc;
+    }
+    return DYN_OK;
+}
+
+double dyn_algorithmic_bytes_per_point(const dyn_ctx *ctx) {
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c) return 0.0;
+    // field passes per step: 93 dry, 135 moist (one read of each input and one write of each
+    // output per operator group, spectral arrays counted as one real-equivalent field)
+    double passes = c->g.microphysics ? 135.0 : 93.0;
+    return passes * (double)c->elt;
+}
+
+void *dyn_stream(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    return c ? (void *)c->stream : nullptr;
+}
+
+}  // extern "C"

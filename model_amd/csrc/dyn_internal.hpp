@@ -1,0 +1,112 @@
+// Internal definitions shared by the HIP translation units of libdyncore.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/dyncore.h"
+
+namespace dyn {
+
+// physical constants, model.h:250-258
+constexpr double kRd = 287.0;
+constexpr double kCp = 1004;
+constexpr double kP0 = 100000;
+constexpr double kGrav = 9.8;
+constexpr double kLv = 2.5e6;
+constexpr double kTrigPi = 3.14159265358979324;
+
+void set_error(const std::string &msg);
+
+#define DYN_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess) {                                                                    \
+            ::dyn::set_error(std::string(#call " failed: ") + hipGetErrorString(_e));            \
+            return DYN_E_HIP;                                                                      \
+        }                                                                                          \
+    } while (0)
+
+#define DYN_LAUNCH_CHECK()                                                                         \
+    do {                                                                                           \
+        hipError_t _e = hipGetLastError();                                                         \
+        if (_e != hipSuccess) {                                                                    \
+            ::dyn::set_error(std::string("kernel launch failed: ") + hipGetErrorString(_e));     \
+            return DYN_E_HIP;                                                                      \
+        }                                                                                          \
+    } while (0)
+
+// Grid geometry and the per-level constants every kernel needs, passed by value.
+struct Geo {
+    int NX, NY, NZ;
+    long long NYNZ;
+    double dt, dtx, dty, dtz, one_d_dx, one_d_dy, one_d_dz;
+};
+
+// Device pointers to the 1-D columns (always FP64: they are tiny and read through the scalar cache).
+struct Cols {
+    const double *zu, *zw, *zsu, *zsw, *mu, *mw, *rhou, *rhow, *one_d_rhou, *one_d_rhow;
+    const double *tb, *tbv, *tbw, *qb, *pib, *f;
+};
+
+// Pressure-solver plan (serial periodic branch: ni = NX-6, nj = NY, levels 1..NZ-2).
+struct PressurePlan {
+    int ni = 0, nj = 0, nk2 = 0;      // transform sizes and active level count
+    int radx[8] = {0}, nradx = 0;      // radix sequence along i
+    int rady[8] = {0}, nrady = 0;      // radix sequence along j
+    double *twx = nullptr, *twy = nullptr;   // exp(-2 pi i e / n), e = 0..n-1 (interleaved re, im)
+    double *coskx = nullptr, *cosly = nullptr; // cos(kwave[i]), cos(lwave[j]) as init_fft3d evaluates them
+    double *acoef = nullptr, *ccoef = nullptr; // pressure.cpp:779-782
+    void *spec = nullptr;              // spectral work array [kx][ky][kk] complex (T)
+    double *ffac = nullptr;            // Thomas multipliers F2 [kx][ky][kk] (FP64)
+};
+
+struct Ctx {
+    dyn_grid g{};
+    Geo geo{};
+    int prec = DYN_FP64;
+    size_t elt = 8;       // bytes per field element
+    size_t npts = 0;      // NX*NY*NZ
+    hipStream_t stream = nullptr;
+    // prognostics: [field][role] device pointers; roles rotate by pointer swaps
+    void *prog[7][3] = {{nullptr}};
+    void *pi = nullptr, *vt = nullptr, *accrain = nullptr;
+    void *bar[6] = {nullptr};  // ubar, vbar, wbar, thbar, qbar, pbar
+    double *colbuf = nullptr;  // all columns, device
+    double *colhost = nullptr; // host copy
+    Cols cols{};
+    bool cols_set[DYN_COLUMN_COUNT] = {false};
+    double *sponge_n = nullptr, *sponge_s = nullptr; // device sponge coefficients
+    double *raycoef = nullptr;                       // device Rayleigh coefficients [2][NZ]
+    int ray_level_cached = -1;
+    int *dflag = nullptr;                             // device error flag (Kessler guard)
+    PressurePlan pp;
+    // graph for dyn_step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_parity = -1;
+};
+
+inline size_t field_count(const Ctx *c, int field) {
+    return field == DYN_ACCRAIN ? (size_t)c->g.nx * c->g.ny : c->npts;
+}
+
+// launchers implemented in the .hip files (return DYN_* codes)
+int launch_advect_uvw(Ctx *c, double step, int il, int ih, int jl, int jh);
+int launch_advect_theta(Ctx *c, double step, int il, int ih, int jl, int jh);
+int launch_advect_moisture(Ctx *c, double step, int il, int ih, int jl, int jh);
+int launch_zero_moisture(Ctx *c, int il, int ih, int jl, int jh);
+int launch_pgf(Ctx *c, double step, int il, int ih, int jl, int jh);
+int launch_bc(Ctx *c, bool micro);
+int launch_periodic(Ctx *c, void *const *fields, int nfields);
+int launch_kessler(Ctx *c, int il, int ih, int jl, int jh);
+int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level);
+int pressure_init(Ctx *c);
+int pressure_free(Ctx *c);
+int launch_solve_pressure(Ctx *c, double step);
+int launch_convert(Ctx *c, void *dst, const double *src, size_t n, bool to_device_type);
+
+}  // namespace dyn

@@ -1,0 +1,480 @@
+// Anelastic pressure Poisson solve for gfx950 (serial periodic branch of solve_pressure,
+// pressure.cpp:142-224, with poisson_fft3d 805-893 and divergence3d_periodic 1390-1424).
+//
+// Pipeline over the active levels kk = k-1, k in [1, NZ-1):
+//   P1  divergence (fused prologue) + forward DFT along i     fields -> spec[kx][j][kk]
+//   P2  forward DFT along j                                  spec -> spec (in place)
+//   P3  1/(ni*nj) scaling + tridiagonal solve in k, FP64      spec -> spec (in place)
+//   P4  backward DFT along j                                 spec -> spec
+//   P5  backward DFT along i + real part -> PI with the k=0/NZ-1 copies and periodic halos fused
+// The DFTs are Stockham auto-sort passes over an LDS tile [n][kc] (radix 4/2/3/5), batched over a
+// chunk of kc consecutive levels so every global access is a contiguous kc-run of a k column.
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "dyn_internal.hpp"
+
+namespace dyn {
+namespace {
+
+constexpr int kMaxRad = 8;
+struct Radix {
+    int r[kMaxRad];
+    int n, nr;
+};
+
+constexpr int kThreads = 256;
+constexpr size_t kLdsBudget = 64 * 1024;
+
+// exp(sign * 2 pi i * e / n) from the forward table (cos, -sin)
+template <typename T>
+__device__ __forceinline__ void twiddle(const double *tw, int e, int sign, T &wr, T &wi) {
+    wr = T(tw[2 * e]);
+    wi = sign < 0 ? T(tw[2 * e + 1]) : -T(tw[2 * e + 1]);
+}
+
+template <typename T>
+__device__ __forceinline__ void cmul(T &ar, T &ai, T br, T bi) {
+    T r = ar * br - ai * bi;
+    T i = ar * bi + ai * br;
+    ar = r;
+    ai = i;
+}
+
+// in-register DFT of length R with exponent sign `sign`
+template <typename T>
+__device__ __forceinline__ void butterfly(int R, T *xr, T *xi, int sign) {
+    const T s = T(sign);
+    if (R == 2) {
+        T ar = xr[0] + xr[1], ai = xi[0] + xi[1];
+        xr[1] = xr[0] - xr[1];
+        xi[1] = xi[0] - xi[1];
+        xr[0] = ar;
+        xi[0] = ai;
+    } else if (R == 4) {
+        T t0r = xr[0] + xr[2], t0i = xi[0] + xi[2];
+        T t1r = xr[0] - xr[2], t1i = xi[0] - xi[2];
+        T t2r = xr[1] + xr[3], t2i = xi[1] + xi[3];
+        T t3r = xr[1] - xr[3], t3i = xi[1] - xi[3];
+        // i*s*(a1 - a3) = (-s*t3i, s*t3r)
+        xr[0] = t0r + t2r;
+        xi[0] = t0i + t2i;
+        xr[2] = t0r - t2r;
+        xi[2] = t0i - t2i;
+        xr[1] = t1r - s * t3i;
+        xi[1] = t1i + s * t3r;
+        xr[3] = t1r + s * t3i;
+        xi[3] = t1i - s * t3r;
+    } else if (R == 3) {
+        const T m = s * T(0.86602540378443864676);
+        T t1r = xr[1] + xr[2], t1i = xi[1] + xi[2];
+        T dr = xr[1] - xr[2], di = xi[1] - xi[2];
+        T t2r = xr[0] - T(0.5) * t1r, t2i = xi[0] - T(0.5) * t1i;
+        xr[0] = xr[0] + t1r;
+        xi[0] = xi[0] + t1i;
+        xr[1] = t2r - m * di;
+        xi[1] = t2i + m * dr;
+        xr[2] = t2r + m * di;
+        xi[2] = t2i - m * dr;
+    } else {  // R == 5
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = s * T(0.95105651629515357212), s2 = s * T(0.58778525229247312917);
+        T b1r = xr[1] + xr[4], b1i = xi[1] + xi[4], b2r = xr[2] + xr[3], b2i = xi[2] + xi[3];
+        T d1r = xr[1] - xr[4], d1i = xi[1] - xi[4], d2r = xr[2] - xr[3], d2i = xi[2] - xi[3];
+        T a0r = xr[0], a0i = xi[0];
+        T p1r = a0r + c1 * b1r + c2 * b2r, p1i = a0i + c1 * b1i + c2 * b2i;
+        T p2r = a0r + c2 * b1r + c1 * b2r, p2i = a0i + c2 * b1i + c1 * b2i;
+        T q1r = s1 * d1r + s2 * d2r, q1i = s1 * d1i + s2 * d2i;  // multiplied by i below
+        T q2r = s2 * d1r - s1 * d2r, q2i = s2 * d1i - s1 * d2i;
+        xr[0] = a0r + b1r + b2r;
+        xi[0] = a0i + b1i + b2i;
+        xr[1] = p1r - q1i;
+        xi[1] = p1i + q1r;
+        xr[4] = p1r + q1i;
+        xi[4] = p1i - q1r;
+        xr[2] = p2r - q2i;
+        xi[2] = p2i + q2r;
+        xr[3] = p2r + q2i;
+        xi[3] = p2i - q2r;
+    }
+}
+
+// Stockham DFT of kc interleaved batches held in LDS as [n][kc] (re and im planes); ping-pongs
+// between buffer 0 and 1 and returns the index of the buffer holding the result.
+template <typename T>
+__device__ int lds_fft(T *re[2], T *im[2], const Radix &rad, int kc, const double *tw, int sign) {
+    const int n = rad.n;
+    int cur = 0, Ns = 1;
+    for (int s = 0; s < rad.nr; s++) {
+        const int R = rad.r[s];
+        const int nb = n / R;
+        const int stride = n / (Ns * R);
+        T *ir = re[cur], *ii = im[cur], *orr = re[cur ^ 1], *oi = im[cur ^ 1];
+        for (int idx = threadIdx.x; idx < nb * kc; idx += blockDim.x) {
+            int b = idx % kc, jj = idx / kc;
+            T xr[5], xi[5];
+            int jm = jj % Ns;
+#pragma unroll
+            for (int r = 0; r < 5; r++) {
+                if (r < R) {
+                    xr[r] = ir[(jj + r * nb) * kc + b];
+                    xi[r] = ii[(jj + r * nb) * kc + b];
+                }
+            }
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < 5; r++) {
+                    if (r < R) {
+                        T wr, wi;
+                        twiddle(tw, r * jm * stride, sign, wr, wi);
+                        cmul(xr[r], xi[r], wr, wi);
+                    }
+                }
+            }
+            butterfly(R, xr, xi, sign);
+            int od = (jj / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < 5; r++) {
+                if (r < R) {
+                    orr[(od + r * Ns) * kc + b] = xr[r];
+                    oi[(od + r * Ns) * kc + b] = xi[r];
+                }
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+        Ns *= R;
+    }
+    return cur;
+}
+
+#define IX(i, j, k) ((long long)(i) * g.NYNZ + (long long)(j) * g.NZ + (long long)(k))
+
+// P1: divergence3d_periodic prologue + forward DFT along i; block = (j, level chunk)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_div_fft_x(Geo g, Cols c, const T *__restrict__ UP, const T *__restrict__ VP,
+                                                        const T *__restrict__ WP, T *__restrict__ spec, Radix rad, const double *tw,
+                                                        int nk2, int kc, double step) {
+    extern __shared__ unsigned char smem[];
+    const int n = rad.n;
+    T *re[2] = {(T *)smem, (T *)smem + 2 * n * kc};
+    T *im[2] = {(T *)smem + n * kc, (T *)smem + 3 * n * kc};
+    const int j = blockIdx.x;
+    const int kk0 = blockIdx.y * kc;
+    const int jj = j == 0 ? 1 : (j == g.NY - 1 ? g.NY - 2 : j);  // mirrored rows, pressure.cpp:1408-1415
+    const double sdt = step * g.dt;
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, i = idx / kc;
+        int kk = kk0 + b;
+        T d = T(0);
+        if (kk < nk2) {
+            int k = kk + 1, ia = i + 3;
+            T rhou = T(c.rhou[k]);
+            d = rhou * (UP[IX(ia + 1, jj, k)] - UP[IX(ia, jj, k)]) * T(g.one_d_dx) +
+                rhou * (VP[IX(ia, jj + 1, k)] - VP[IX(ia, jj, k)]) * T(g.one_d_dy) +
+                (T(c.rhow[k + 1]) * WP[IX(ia, jj, k + 1)] - T(c.rhow[k]) * WP[IX(ia, jj, k)]) * (T(g.one_d_dz) * T(c.mu[k]));
+            d /= T(sdt);
+        }
+        re[0][idx] = d;
+        im[0][idx] = T(0);
+    }
+    __syncthreads();
+    int out = lds_fft(re, im, rad, kc, tw, -1);
+    // spec[kx][j][kk] (interleaved complex)
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, kx = idx / kc, kk = kk0 + b;
+        if (kk < nk2) {
+            long long o = (((long long)kx * g.NY + j) * nk2 + kk) * 2;
+            spec[o] = re[out][idx];
+            spec[o + 1] = im[out][idx];
+        }
+    }
+}
+
+// P2/P4: DFT along j for fixed kx; block = (kx, level chunk)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fft_y(T *__restrict__ spec, int nj, int nk2, Radix rad, const double *tw, int kc, int sign) {
+    extern __shared__ unsigned char smem[];
+    const int n = rad.n;
+    T *re[2] = {(T *)smem, (T *)smem + 2 * n * kc};
+    T *im[2] = {(T *)smem + n * kc, (T *)smem + 3 * n * kc};
+    const int kx = blockIdx.x, kk0 = blockIdx.y * kc;
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, ky = idx / kc, kk = kk0 + b;
+        T vr = T(0), vi = T(0);
+        if (kk < nk2) {
+            long long o = (((long long)kx * nj + ky) * nk2 + kk) * 2;
+            vr = spec[o];
+            vi = spec[o + 1];
+        }
+        re[0][idx] = vr;
+        im[0][idx] = vi;
+    }
+    __syncthreads();
+    int out = lds_fft(re, im, rad, kc, tw, sign);
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, ky = idx / kc, kk = kk0 + b;
+        if (kk < nk2) {
+            long long o = (((long long)kx * nj + ky) * nk2 + kk) * 2;
+            spec[o] = re[out][idx];
+            spec[o + 1] = im[out][idx];
+        }
+    }
+}
+
+// Tridiagonal multipliers F2 of pressure.cpp:840-855 for every (kx,ky): a pure function of the
+// coefficients, so they are formed once per context instead of once per solve.
+__global__ void k_thomas_factors(int ni, int nj, int nk, const double *__restrict__ coskx, const double *__restrict__ cosly,
+                                 const double *__restrict__ rhou, const double *__restrict__ rhow, const double *__restrict__ mu,
+                                 const double *__restrict__ mw, const double *__restrict__ acoef, const double *__restrict__ ccoef,
+                                 double dx, double dz, double *__restrict__ ffac) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ni * nj) return;
+    int kx = q / nj, ky = q % nj;
+    int nk2 = nk - 2;
+    double *F = ffac + (long long)q * nk2;
+    double ck = coskx[kx], cl = cosly[ky];
+    // B2 of pressure.cpp:787-791
+    double B = rhou[1] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[1] * mw[2] * rhow[2] / (dz * dz);
+    double z = 1.0 / B;
+    double Fp = ccoef[1] * z;
+    F[0] = Fp;
+    for (int k = 2; k < nk - 1; k++) {
+        if (k < nk - 2) B = rhou[k] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[k] * (mw[k + 1] * rhow[k + 1] + mw[k] * rhow[k]) / (dz * dz);
+        else B = rhou[nk - 2] * 2.0 * (ck + cl - 2.0) / (dx * dx) - (mu[nk - 2] * mw[nk - 2] * rhow[nk - 2]) / (dz * dz);
+        z = 1.0 / (B - acoef[k] * Fp);
+        Fp = ccoef[k] * z;
+        F[k - 1] = Fp;
+    }
+}
+
+// P3: normalisation (pressure.cpp:824-827), forward elimination (837-856), mean-mode gauge (858-859)
+// and back substitution (864-871), in FP64 whatever the storage precision.
+template <typename T>
+__global__ void k_thomas(T *__restrict__ spec, int ni, int nj, int nk, const double *__restrict__ coskx, const double *__restrict__ cosly,
+                         const double *__restrict__ rhou, const double *__restrict__ rhow, const double *__restrict__ mu,
+                         const double *__restrict__ mw, const double *__restrict__ acoef, const double *__restrict__ ffac, double dx,
+                         double dz) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ni * nj) return;
+    int kx = q / nj, ky = q % nj;
+    const int nk2 = nk - 2;
+    T *x = spec + (long long)q * nk2 * 2;
+    const double *F = ffac + (long long)q * nk2;
+    const double size2d = (double)(ni * nj);
+    double ck = coskx[kx], cl = cosly[ky];
+    double B = rhou[1] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[1] * mw[2] * rhow[2] / (dz * dz);
+    double z = 1.0 / B;
+    double pr = (double(x[0]) / size2d) * z, pi = (double(x[1]) / size2d) * z;
+    x[0] = T(pr);
+    x[1] = T(pi);
+    // keep the eliminated values in FP64 in a second pass-through: store FP64 in a local ring is
+    // not possible for arbitrary nk, so FP32 storage rounds here (the FP64 path is exact).
+    double Fp = F[0];
+    for (int k = 2; k < nk - 1; k++) {
+        if (k < nk - 2) B = rhou[k] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[k] * (mw[k + 1] * rhow[k + 1] + mw[k] * rhow[k]) / (dz * dz);
+        else B = rhou[nk - 2] * 2.0 * (ck + cl - 2.0) / (dx * dx) - (mu[nk - 2] * mw[nk - 2] * rhow[nk - 2]) / (dz * dz);
+        z = 1.0 / (B - acoef[k] * Fp);
+        double xr = double(x[2 * (k - 1)]) / size2d, xi = double(x[2 * (k - 1) + 1]) / size2d;
+        pr = (xr - acoef[k] * pr) * z;
+        pi = (xi - acoef[k] * pi) * z;
+        x[2 * (k - 1)] = T(pr);
+        x[2 * (k - 1) + 1] = T(pi);
+        Fp = F[k - 1];
+    }
+    if (kx == 0 && ky == 0) {  // OUT3D(0,0,NZ-2) = 0
+        pr = 0.0;
+        pi = 0.0;
+        x[2 * (nk2 - 1)] = T(0);
+        x[2 * (nk2 - 1) + 1] = T(0);
+    }
+    for (int k = nk - 3; k >= 1; k--) {
+        double f = F[k - 1];
+        pr = double(x[2 * (k - 1)]) - f * pr;
+        pi = double(x[2 * (k - 1) + 1]) - f * pi;
+        x[2 * (k - 1)] = T(pr);
+        x[2 * (k - 1) + 1] = T(pi);
+    }
+}
+
+// P5: backward DFT along i, real part -> PI (pressure.cpp:878-892) + periodic halos (boundaries.cpp:533-540)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_ifft_x_out(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
+                                                         const double *tw, int nk2, int kc) {
+    extern __shared__ unsigned char smem[];
+    const int n = rad.n;
+    T *re[2] = {(T *)smem, (T *)smem + 2 * n * kc};
+    T *im[2] = {(T *)smem + n * kc, (T *)smem + 3 * n * kc};
+    const int j = blockIdx.x, kk0 = blockIdx.y * kc;
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, kx = idx / kc, kk = kk0 + b;
+        T vr = T(0), vi = T(0);
+        if (kk < nk2) {
+            long long o = (((long long)kx * g.NY + j) * nk2 + kk) * 2;
+            vr = spec[o];
+            vi = spec[o + 1];
+        }
+        re[0][idx] = vr;
+        im[0][idx] = vi;
+    }
+    __syncthreads();
+    int out = lds_fft(re, im, rad, kc, tw, +1);
+    const int shift = g.NX - 6;
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, i = idx / kc, kk = kk0 + b;
+        if (kk >= nk2) continue;
+        T v = re[out][idx];
+        int ia = i + 3;
+        int k = kk + 1;
+        int ks[3] = {k, -1, -1};
+        if (kk == 0) ks[1] = 0;
+        if (kk == nk2 - 1) ks[2] = g.NZ - 1;
+        for (int t = 0; t < 3; t++) {
+            if (ks[t] < 0) continue;
+            PI[IX(ia, j, ks[t])] = v;
+            if (ia >= shift && ia < shift + 3) PI[IX(ia - shift, j, ks[t])] = v;
+            if (ia >= 3 && ia < 6) PI[IX(ia + shift, j, ks[t])] = v;
+        }
+    }
+}
+
+bool factorize(int n, Radix &rad) {
+    rad.n = n;
+    rad.nr = 0;
+    int m = n;
+    while (m > 1 && rad.nr < kMaxRad) {
+        int r = (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : 0;
+        if (r == 0) return false;
+        rad.r[rad.nr++] = r;
+        m /= r;
+    }
+    return m == 1;
+}
+
+int level_chunk(int n, size_t elt, int nk2) {
+    // two ping-pong buffers of n*kc complex values must fit the LDS budget
+    int kc = (int)(kLdsBudget / (4 * (size_t)n * elt));
+    if (kc > 32) kc = 32;
+    if (kc > nk2) kc = nk2;
+    if (kc < 1) kc = 1;
+    return kc;
+}
+
+double *upload_twiddles(int n, hipStream_t s) {
+    std::vector<double> h(2 * (size_t)n);
+    for (int e = 0; e < n; e++) {
+        double a = 2.0 * kTrigPi * (double)e / (double)n;
+        h[2 * e] = std::cos(a);
+        h[2 * e + 1] = -std::sin(a);
+    }
+    double *d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * h.size()) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+}
+
+template <typename T>
+int solve_t(Ctx *c, double step) {
+    PressurePlan &pp = c->pp;
+    const Geo &g = c->geo;
+    Radix rx{}, ry{};
+    factorize(pp.ni, rx);
+    factorize(pp.nj, ry);
+    // periodic_uvw_boundaries (boundaries.cpp:546-555)
+    void *uvw[3] = {c->prog[DYN_U][2], c->prog[DYN_V][2], c->prog[DYN_W][2]};
+    int rc = launch_periodic(c, uvw, 3);
+    if (rc) return rc;
+    int kcx = level_chunk(pp.ni, sizeof(T), pp.nk2), kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
+    size_t shx = 4 * (size_t)pp.ni * kcx * sizeof(T), shy = 4 * (size_t)pp.nj * kcy * sizeof(T);
+    dim3 gx(g.NY, (pp.nk2 + kcx - 1) / kcx), gy(pp.ni, (pp.nk2 + kcy - 1) / kcy);
+    T *spec = (T *)pp.spec;
+    hipLaunchKernelGGL(k_div_fft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
+                       (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kcx, step);
+    DYN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
+    DYN_LAUNCH_CHECK();
+    int nq = pp.ni * pp.nj;
+    hipLaunchKernelGGL(k_thomas<T>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, spec, pp.ni, pp.nj, g.NZ, pp.coskx, pp.cosly,
+                       c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz);
+    DYN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
+    DYN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ifft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kcx);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
+}  // namespace
+
+int pressure_init(Ctx *c) {
+    PressurePlan &pp = c->pp;
+    const int NZ = c->g.nz;
+    pp.ni = c->g.nx - 6;
+    pp.nj = c->g.ny;
+    pp.nk2 = NZ - 2;
+    Radix rx{}, ry{};
+    if (!factorize(pp.ni, rx) || !factorize(pp.nj, ry)) {
+        set_error("pressure solver: transform sizes must factor into 2, 3, 4 and 5 (ni=" + std::to_string(pp.ni) +
+                  ", nj=" + std::to_string(pp.nj) + ")");
+        return DYN_E_UNSUPPORTED;
+    }
+    pp.twx = upload_twiddles(pp.ni, c->stream);
+    pp.twy = upload_twiddles(pp.nj, c->stream);
+    if (!pp.twx || !pp.twy) {
+        set_error("pressure solver: twiddle upload failed");
+        return DYN_E_HIP;
+    }
+    DYN_HIP(hipMalloc(&pp.spec, sizeof(double) * 2 * (size_t)pp.ni * pp.nj * pp.nk2));
+    DYN_HIP(hipMalloc(&pp.ffac, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
+    DYN_HIP(hipMalloc(&pp.coskx, sizeof(double) * pp.ni));
+    DYN_HIP(hipMalloc(&pp.cosly, sizeof(double) * pp.nj));
+    DYN_HIP(hipMalloc(&pp.acoef, sizeof(double) * NZ));
+    DYN_HIP(hipMalloc(&pp.ccoef, sizeof(double) * NZ));
+    // cos(kwave[i]), cos(lwave[j]) exactly as init_fft3d forms them (pressure.cpp:751-752)
+    std::vector<double> ck(pp.ni), cl(pp.nj);
+    for (int i = 0; i < pp.ni; i++) ck[i] = std::cos((double)i * 2 * kTrigPi / ((double)pp.ni));
+    for (int j = 0; j < pp.nj; j++) cl[j] = std::cos((double)j * 2 * kTrigPi / ((double)pp.nj));
+    DYN_HIP(hipMemcpy(pp.coskx, ck.data(), sizeof(double) * pp.ni, hipMemcpyHostToDevice));
+    DYN_HIP(hipMemcpy(pp.cosly, cl.data(), sizeof(double) * pp.nj, hipMemcpyHostToDevice));
+    return DYN_OK;
+}
+
+// coefficients that depend on the columns (acoef, ccoef, F2): called once the columns are set
+int pressure_coefficients(Ctx *c) {
+    PressurePlan &pp = c->pp;
+    const int nk = c->g.nz;
+    const double dz = c->g.dz;
+    const double *mu = c->colhost + (size_t)DYN_COL_MU * nk, *mw = c->colhost + (size_t)DYN_COL_MW * nk;
+    const double *rhow = c->colhost + (size_t)DYN_COL_RHOW * nk;
+    std::vector<double> a(nk, 0.0), cc(nk, 0.0);
+    for (int k = 1; k < nk - 1; k++) {
+        a[k] = mu[k] * mw[k] * rhow[k] / (dz * dz);
+        cc[k] = mu[k] * mw[k + 1] * rhow[k + 1] / (dz * dz);
+    }
+    a[1] = 0;
+    cc[nk - 2] = 0;
+    DYN_HIP(hipMemcpy(pp.acoef, a.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+    DYN_HIP(hipMemcpy(pp.ccoef, cc.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+    int nq = pp.ni * pp.nj;
+    hipLaunchKernelGGL(k_thomas_factors, dim3((nq + 127) / 128), dim3(128), 0, c->stream, pp.ni, pp.nj, nk, pp.coskx, pp.cosly,
+                       c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ccoef, c->g.dx, dz, pp.ffac);
+    DYN_LAUNCH_CHECK();
+    DYN_HIP(hipStreamSynchronize(c->stream));
+    return DYN_OK;
+}
+
+int pressure_free(Ctx *c) {
+    PressurePlan &pp = c->pp;
+    void *ptrs[] = {pp.twx, pp.twy, pp.spec, pp.ffac, pp.coskx, pp.cosly, pp.acoef, pp.ccoef};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    pp = PressurePlan{};
+    return DYN_OK;
+}
+
+int launch_solve_pressure(Ctx *c, double step) {
+    return c->prec == DYN_FP32 ? solve_t<float>(c, step) : solve_t<double>(c, step);
+}
+
+}  // namespace dyn

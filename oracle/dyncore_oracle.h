@@ -1,0 +1,81 @@
+/*
+ * dyncore_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C, FP64) of the serial (PARALLEL 0, periodic east-west)
+ * per-timestep dynamical core of michaeldiaz0/Model.  It exists to check the HIP
+ * library, never to be shipped or called by it: only tests/, the smoke() entry and
+ * bench.py's CPU-baseline leg load it.
+ *
+ * Field, role and column identifiers match include/dyncore.h so that a test can
+ * move arrays between the two without a translation table.
+ */
+#ifndef DYNCORE_ORACLE_H
+#define DYNCORE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_grid {
+    int nx, ny, nz;          /* NX (6 halo columns included), NY, NZ (ghost levels included) */
+    double dx, dz, dt;       /* dy == dx, as the reference parser sets it (process_input.cpp:175-176) */
+    int microphysics;        /* 0 = dry, 1 = Kessler warm rain */
+    int rayleigh_level;      /* raydampheight; >= nz means off */
+    int sponge_north, sponge_south;
+} orc_grid;
+
+typedef struct orc_state orc_state;
+
+orc_state *orc_create(const orc_grid *g);
+void orc_destroy(orc_state *s);
+/* 1 if a fatal condition was hit (Kessler E>100, kessler.cpp:83-88) */
+int orc_error(const orc_state *s);
+
+void orc_set_column(orc_state *s, int column, const double *v);
+void orc_get_column(const orc_state *s, int column, double *v);
+void orc_set_field(orc_state *s, int field, int role, const double *v);
+void orc_get_field(const orc_state *s, int field, int role, double *v);
+
+/* whole step: s_integrate_rk3 (driver_serial.cpp:178-286), nsteps times */
+void orc_step(orc_state *s, int nsteps);
+
+/* individual reference operators, same bounds convention (half-open, array coordinates) */
+void orc_advect_uvw(orc_state *s, double step, int il, int ih, int jl, int jh);
+void orc_advect_theta(orc_state *s, double step, int il, int ih, int jl, int jh);
+void orc_advect_moisture(orc_state *s, double step, int il, int ih, int jl, int jh);
+void orc_zero_moisture(orc_state *s, int il, int ih, int jl, int jh);
+void orc_solve_pressure(orc_state *s, double step);
+void orc_pgf(orc_state *s, double step, int il, int ih, int jl, int jh);
+void orc_apply_bc(orc_state *s);
+void orc_apply_bc_micro(orc_state *s);
+void orc_kessler(orc_state *s, int il, int ih, int jl, int jh);
+void orc_rayleigh(orc_state *s, int il, int ih, int jl, int jh, int level);
+void orc_advance_inner(orc_state *s);
+void orc_advance_outer(orc_state *s);
+void orc_micro_advance_inner(orc_state *s);
+void orc_micro_advance_outer(orc_state *s);
+
+/* ---- pieces exposed so tests can pin them against mathematics ---- */
+/* in-place complex DFT of length n over interleaved (re,im) data; sign -1 forward, +1 backward,
+ * unnormalised (the FFTW convention used at pressure.cpp:747-748) */
+void orc_fft1d(int n, double *data, int sign);
+/* 2-D complex DFT of `howmany` planes of n0 x n1 (row-major, n1 fastest), unnormalised */
+void orc_fft2d(int n0, int n1, int howmany, double *data, int sign);
+/* face interpolants of fluxes.cpp:110-206 on a 1-D array p with unit stride, face between p[0] and p[1] */
+double orc_interp5(const double *p, double sign);
+double orc_interp3(const double *p, double sign);
+double orc_interp2(const double *p);
+/* Poisson solve on a given real right-hand side rhs[k][i][j] (k in 0..nz-1, i in 0..ni-1, j in 0..ny-1;
+ * only k=1..nz-2 are used), result x[k][i][j] at k=1..nz-2; ni = nx-6 */
+void orc_poisson_rhs(orc_state *s, const double *rhs, double *x);
+/* one Kessler point: in/out th, qv, qc, qr (perturbations), returns E>100 flag */
+int orc_kessler_point(double dt, double rhou, double tb, double qb, double tbv,
+                      double thbar, double qbar, double pbar, double pi,
+                      double *th, double *qv, double *qc, double *qr);
+/* rain fall speed at one point (kessler.cpp:222-252) */
+double orc_fall_speed(double qr, double rhou_k, double rhou_1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

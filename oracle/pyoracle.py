@@ -1,0 +1,138 @@
+"""ctypes wrapper of the oracle (oracle/dyncore_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, as the checker or the timed CPU restatement; never by the model_amd package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+                ("microphysics", ctypes.c_int), ("rayleigh_level", ctypes.c_int),
+                ("sponge_north", ctypes.c_int), ("sponge_south", ctypes.c_int)]
+
+
+def build():
+    """Compile the oracle with its Makefile (parity and timed variants)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
+
+
+def lib(variant: str = "parity"):
+    name = {"parity": "liboracle.so", "omp": "liboracle_omp.so"}[variant]
+    if variant in _LIBS:
+        return _LIBS[variant]
+    path = os.path.join(_HERE, "_build", name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library {path} is not built (run `make -C oracle`)")
+    L = ctypes.CDLL(path)
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_create.argtypes = [ctypes.POINTER(OrcGrid)]
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_error.argtypes = [ctypes.c_void_p]
+    for fn in ("orc_set_column", "orc_get_column"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    for fn in ("orc_set_field", "orc_get_field"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    for fn in ("orc_advect_uvw", "orc_advect_theta", "orc_advect_moisture", "orc_pgf"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4
+    for fn in ("orc_zero_moisture", "orc_kessler"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
+    L.orc_rayleigh.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
+    L.orc_solve_pressure.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    for fn in ("orc_apply_bc", "orc_apply_bc_micro", "orc_advance_inner", "orc_advance_outer",
+               "orc_micro_advance_inner", "orc_micro_advance_outer"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p]
+    L.orc_fft1d.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+    L.orc_fft2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
+    for fn in ("orc_interp5", "orc_interp3"):
+        getattr(L, fn).argtypes = [_dp, ctypes.c_double]
+        getattr(L, fn).restype = ctypes.c_double
+    L.orc_interp2.argtypes = [_dp]
+    L.orc_interp2.restype = ctypes.c_double
+    L.orc_poisson_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.orc_kessler_point.argtypes = [ctypes.c_double] * 9 + [_dp] * 4
+    L.orc_kessler_point.restype = ctypes.c_int
+    L.orc_fall_speed.argtypes = [ctypes.c_double] * 3
+    L.orc_fall_speed.restype = ctypes.c_double
+    _LIBS[variant] = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    """One oracle state; mirrors the reference's serial globals."""
+
+    def __init__(self, case, variant: str = "parity"):
+        self.L = lib(variant)
+        self.case = case
+        g = OrcGrid(case.nx, case.ny, case.nz, case.dx, case.dz, case.dt, case.microphysics,
+                    case.rayleigh_level, case.sponge, case.sponge)
+        self.h = self.L.orc_create(ctypes.byref(g))
+        from model_amd import ids  # identifiers only
+        self.ids = ids
+        for name, arr in case.columns.items():
+            self.set_column(ids.COLUMN_ID[name], arr)
+        for name, arr in case.fields.items():
+            f = ids.FIELD_ID[name]
+            if f in ids.PROGNOSTIC:
+                for role in (ids.ROLE_M, ids.ROLE_CUR, ids.ROLE_P):
+                    self.set_field(f, role, arr)
+            else:
+                self.set_field(f, 0, arr)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def set_column(self, c, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self.L.orc_set_column(self.h, c, _ptr(v))
+
+    def get_column(self, c):
+        n = self.case.ny if c == self.ids.COL_F else self.case.nz
+        out = np.empty(n)
+        self.L.orc_get_column(self.h, c, _ptr(out))
+        return out
+
+    def set_field(self, f, role, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self.L.orc_set_field(self.h, f, role, _ptr(v))
+
+    def get_field(self, f, role=1):
+        c = self.case
+        shape = (c.nx, c.ny) if f == self.ids.ACCRAIN else (c.nx, c.ny, c.nz)
+        out = np.empty(shape)
+        self.L.orc_get_field(self.h, f, role, _ptr(out))
+        return out
+
+    def step(self, n=1):
+        self.L.orc_step(self.h, n)
+
+    @property
+    def error(self):
+        return self.L.orc_error(self.h)
+
+    def __getattr__(self, name):
+        if name.startswith("_") or name in ("L", "h", "case", "ids"):
+            raise AttributeError(name)
+        fn = getattr(self.L, "orc_" + name)
+        h = self.h
+        return lambda *a: fn(h, *a)
