@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Throughput of the anelastic dynamical core on one MI355X.
+
+Workload: the moist Kessler supercell case at 256x256x64 (NX=262 with the periodic halo, NZ with its two
+ghost levels), dt = 2 s, synthetic sounding and bubble built in memory, all fields resident in HBM
+before the timed region.  A step is one full RK3 time step (s_integrate_rk3: three stages of advection,
+pressure solve and boundaries, then Kessler, boundaries and Rayleigh damping), replayed from a hipGraph.
+
+`value` is the FP32 form's gridpoint-steps/s (FP32 fields and stencils, FP64 vertical solve); the FP64
+plain form is measured beside it.  Gridpoints per step = 256*256*64 (halo columns not counted).
+
+The roofline object describes the dominant kernel of the FP32 step: its algorithmic HBM bytes per
+launch (field passes x element size x gridpoints processed) divided by its mean launch time, measured
+with HIP events on the context's own stream over an instrumented run of the same step sequence.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gridpoint-steps/s at 256²×64 FP32 vs FP64 on 1 GPU; adjoint/forward cost ratio"
+PEAK_HBM_GBS = 8000.0
+
+# algorithmic field passes per gridpoint processed, per kernel (one read of each input field and one write of
+# each output field; a half-spectrum complex array counts as one real-equivalent field)
+PASSES = {
+    "advect_uvw": {0: 13, 1: 16}, "advect_theta": 10, "advect_moisture": 16, "fall_speed": 2, "zero_moisture": 7,
+    "pgf": 7, "p1_div_fft_x": 4, "p2_fft_y": 2, "p3_thomas": 2, "p4_ifft_y": 2, "p5_ifft_x_out": 2, "kessler": 12,
+    "rayleigh": 8, "advance_copy": 2, "fused_stage": None,
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_form(case, precision, steps, warmup):
+    from model_amd.core import DynCore
+    g = DynCore(case, precision=precision)
+    g.step(warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    return g, (t1 - t0) * 1e3 / steps
+
+
+def kernel_profile(g, steps):
+    g.timing(1)
+    g.step_reference(steps)
+    g.sync()
+    import ctypes
+    buf = ctypes.create_string_buffer(1 << 16)
+    g.timing_report(buf, len(buf))
+    g.timing(0)
+    rows = []
+    for line in buf.value.decode().strip().splitlines():
+        name, n, ms, units = line.split()
+        rows.append((name, int(n), float(ms), float(units)))
+    return rows
+
+
+def roofline(rows, elt, moist):
+    name, n, ms, units = max(rows, key=lambda r: r[2])
+    passes = PASSES.get(name)
+    if isinstance(passes, dict):
+        passes = passes[moist]
+    per_launch_units = units / n
+    avg_ms = ms / n
+    bytes_per_launch = passes * elt * per_launch_units
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 5),
+            "algorithmic_bytes_per_launch": int(bytes_per_launch), "field_passes_per_point": passes,
+            "points_per_launch": int(per_launch_units)}
+
+
+def cpu_baseline(case, budget_s=20.0):
+    """The oracle (the C restatement of the reference serial path) on the host cores, OpenMP over i-slabs."""
+    from oracle.pyoracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(case, variant="omp")
+    t0 = time.perf_counter()
+    o.step(1)  # first step: also faults the pages in
+    first = time.perf_counter() - t0
+    n = max(1, min(20, int((budget_s - first) / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    rate = case.interior_points * n / dt
+    return {"value": round(rate, 1), "unit": "gridpoint-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the same 256x256x64 moist case (after 1 warm-up step), FP64, "
+                      f"OpenMP over i-slabs, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["both", "fp32", "fp64"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would shard); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from model_amd import configs
+    from model_amd.core import DYN_FP32, DYN_FP64
+
+    case = configs.supercell()
+    npts = case.interior_points
+    log(f"bench: workload {case.name} NX={case.nx} NY={case.ny} NZ={case.nz}, {npts} gridpoints per step")
+    res = {}
+    forms = [("fp32", DYN_FP32), ("fp64", DYN_FP64)] if args.form == "both" else \
+        [(args.form, DYN_FP32 if args.form == "fp32" else DYN_FP64)]
+    handles = {}
+    for name, prec in forms:
+        g, ms = time_form(case, prec, args.steps, args.warmup)
+        res[name] = {"ms_per_step": round(ms, 4), "value": round(npts / (ms * 1e-3), 1)}
+        handles[name] = g
+        log(f"bench: {name} {ms:.3f} ms/step, {res[name]['value']:.4g} gridpoint-steps/s")
+    main_form = forms[0][0]
+    g = handles[main_form]
+    rows = kernel_profile(g, args.profile_steps)
+    for r in sorted(rows, key=lambda r: -r[2]):
+        log(f"bench: kernel {r[0]:16s} launches {r[1]:5d} total {r[2]:9.3f} ms avg {r[2] / r[1] * 1e3:8.1f} us")
+    roof = roofline(rows, 4 if main_form == "fp32" else 8, case.microphysics)
+    out = {
+        "metric": METRIC,
+        "value": res[main_form]["value"],
+        "unit": "gridpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res[main_form]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if main_form == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "moist Kessler supercell, 256x256x64 (NX=262 incl. halo), dx=1 km, dz=250 m, dt=2 s",
+                   "nx": 256, "ny": 256, "nz": 64, "form": main_form, "step": "hipGraph replay of s_integrate_rk3"},
+        "forms": res,
+        "fp32_over_fp64": round(res["fp32"]["value"] / res["fp64"]["value"], 3) if "fp64" in res and "fp32" in res else None,
+        "adjoint_forward_ratio": None,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        log("bench: timing the CPU restatement (oracle) on a bounded sample")
+        out["cpu_baseline"] = cpu_baseline(case)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -108,6 +108,11 @@ int dyn_step(dyn_ctx *ctx, int nsteps);
 /* ---- measurement ------------------------------------------------------------------------- */
 /* HBM bytes the step's algorithm must move per gridpoint (the field-pass count of the operator set) */
 double dyn_algorithmic_bytes_per_point(const dyn_ctx *ctx);
+/* per-kernel HIP-event timing of eager launches (dyn_step_reference and the per-operator calls);
+ * dyn_timing(ctx, 1) clears the records and starts timing, dyn_timing_report writes one line per kernel:
+ * "<name> <launches> <total_ms> <gridpoints processed>" */
+int dyn_timing(dyn_ctx *ctx, int on);
+int dyn_timing_report(dyn_ctx *ctx, char *buf, size_t len);
 /* the HIP stream the context launches on (a hipStream_t) */
 void *dyn_stream(dyn_ctx *ctx);
 

@@ -25,7 +25,8 @@ EXPORTS = [
     "dyn_synchronize", "dyn_advect_uvw", "dyn_advect_theta", "dyn_advect_moisture", "dyn_zero_moisture",
     "dyn_solve_pressure", "dyn_pgf", "dyn_apply_bc", "dyn_apply_bc_micro", "dyn_kessler", "dyn_rayleigh",
     "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner", "dyn_micro_advance_outer",
-    "dyn_step_reference", "dyn_step", "dyn_algorithmic_bytes_per_point", "dyn_stream",
+    "dyn_step_reference", "dyn_step", "dyn_algorithmic_bytes_per_point", "dyn_timing", "dyn_timing_report",
+    "dyn_stream",
 ]
 
 
@@ -74,6 +75,8 @@ def load():
     L.dyn_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_algorithmic_bytes_per_point.argtypes = [ctypes.c_void_p]
     L.dyn_algorithmic_bytes_per_point.restype = ctypes.c_double
+    L.dyn_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dyn_timing_report.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     L.dyn_stream.argtypes = [ctypes.c_void_p]
     L.dyn_stream.restype = ctypes.c_void_p
     _lib = L

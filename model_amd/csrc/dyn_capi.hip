@@ -14,6 +14,30 @@ namespace dyn {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+static hipEvent_t take_event(Ctx *c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+Timed::Timed(Ctx *c, const char *name, double units) : c_(c) {
+    if (!c->timing || c->capturing) return;
+    TimerRec r{name, take_event(c), take_event(c), units};
+    if (!r.a || !r.b) return;
+    (void)hipEventRecord(r.a, c->stream);
+    idx_ = (long)c->recs.size();
+    c->recs.push_back(r);
+}
+
+Timed::~Timed() {
+    if (idx_ >= 0) (void)hipEventRecord(c_->recs[idx_].b, c_->stream);
+}
+
 int pressure_coefficients(Ctx *c);
 
 namespace {
@@ -33,6 +57,7 @@ __global__ void k_clear_flag(int *f) { *f = 0; }
 int device_copy(Ctx *c, void *dst, const void *src) {
     long long n = (long long)c->npts;
     unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    Timed _t(c, "advance_copy", (double)n);
     if (c->prec == DYN_FP32) hipLaunchKernelGGL(k_copy_f, dim3(blocks), dim3(256), 0, c->stream, (float *)dst, (const float *)src, n);
     else hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, c->stream, (double *)dst, (const double *)src, n);
     DYN_LAUNCH_CHECK();
@@ -209,6 +234,11 @@ int dyn_destroy(dyn_ctx *ctx) {
     Ctx *c = as_ctx(ctx);
     if (!c) return DYN_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &r : c->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c->pool) (void)hipEventDestroy(e);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     pressure_free(c);
@@ -559,8 +589,10 @@ int dyn_step(dyn_ctx *ctx, int nsteps) {
             void *saved[7][3];
             memcpy(saved, c->prog, sizeof(saved));
             DYN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            c->capturing = true;
             int rc = reference_step(ctx);
             if (!rc) rc = reference_step(ctx);
+            c->capturing = false;
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &graph);
             if (rc || e != hipSuccess) {
@@ -585,6 +617,61 @@ int dyn_step(dyn_ctx *ctx, int nsteps) {
         int rc = reference_step(ctx);
         if (rc) return rc;
     }
+    return DYN_OK;
+}
+
+int dyn_timing(dyn_ctx *ctx, int on) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    DYN_HIP(hipStreamSynchronize(c->stream));
+    for (auto &r : c->recs) {
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    c->timing = on != 0;
+    return DYN_OK;
+}
+
+int dyn_timing_report(dyn_ctx *ctx, char *buf, size_t len) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    if (!buf || len == 0) {
+        set_error("dyn_timing_report: empty buffer");
+        return DYN_E_ARG;
+    }
+    DYN_HIP(hipStreamSynchronize(c->stream));
+    struct Agg {
+        std::string name;
+        double ms = 0, units = 0;
+        long n = 0;
+    };
+    std::vector<Agg> agg;
+    for (auto &r : c->recs) {
+        float ms = 0;
+        DYN_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        Agg *a = nullptr;
+        for (auto &x : agg)
+            if (x.name == r.name) a = &x;
+        if (!a) {
+            agg.push_back(Agg{r.name});
+            a = &agg.back();
+        }
+        a->ms += ms;
+        a->n += 1;
+        a->units += r.units;
+    }
+    std::string out;
+    char line[256];
+    for (auto &a : agg) {
+        snprintf(line, sizeof(line), "%s %ld %.6f %.0f\n", a.name.c_str(), a.n, a.ms, a.units);
+        out += line;
+    }
+    if (out.size() + 1 > len) {
+        set_error("dyn_timing_report: buffer too small");
+        return DYN_E_ARG;
+    }
+    memcpy(buf, out.c_str(), out.size() + 1);
     return DYN_OK;
 }
 

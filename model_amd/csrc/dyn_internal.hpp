@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/dyncore.h"
 
@@ -64,6 +65,13 @@ struct PressurePlan {
     double *ffac = nullptr;            // Thomas multipliers F2 [kx][ky][kk] (FP64)
 };
 
+// per-kernel HIP-event timing on the context stream (eager launches only)
+struct TimerRec {
+    const char *name;
+    hipEvent_t a, b;
+    double units;  // gridpoints the launch processes
+};
+
 struct Ctx {
     dyn_grid g{};
     Geo geo{};
@@ -88,6 +96,22 @@ struct Ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_parity = -1;
+    // timing
+    bool timing = false;
+    bool capturing = false;
+    std::vector<TimerRec> recs;
+    std::vector<hipEvent_t> pool;
+};
+
+// Records a start event at construction and a stop event at destruction when timing is on.
+class Timed {
+  public:
+    Timed(Ctx *c, const char *name, double units);
+    ~Timed();
+
+  private:
+    Ctx *c_;
+    long idx_ = -1;
 };
 
 inline size_t field_count(const Ctx *c, int field) {

@@ -389,18 +389,33 @@ int solve_t(Ctx *c, double step) {
     size_t shx = 4 * (size_t)pp.ni * kcx * sizeof(T), shy = 4 * (size_t)pp.nj * kcy * sizeof(T);
     dim3 gx(g.NY, (pp.nk2 + kcx - 1) / kcx), gy(pp.ni, (pp.nk2 + kcy - 1) / kcy);
     T *spec = (T *)pp.spec;
-    hipLaunchKernelGGL(k_div_fft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
+    {
+        Timed _t(c, "p1_div_fft_x", (double)pp.ni * pp.nj * pp.nk2);
+        hipLaunchKernelGGL(k_div_fft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
                        (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kcx, step);
+    }
     DYN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
+    {
+        Timed _t(c, "p2_fft_y", (double)pp.ni * pp.nj * pp.nk2);
+        hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
+    }
     DYN_LAUNCH_CHECK();
     int nq = pp.ni * pp.nj;
-    hipLaunchKernelGGL(k_thomas<T>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, spec, pp.ni, pp.nj, g.NZ, pp.coskx, pp.cosly,
+    {
+        Timed _t(c, "p3_thomas", (double)pp.ni * pp.nj * pp.nk2);
+        hipLaunchKernelGGL(k_thomas<T>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, spec, pp.ni, pp.nj, g.NZ, pp.coskx, pp.cosly,
                        c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz);
+    }
     DYN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
+    {
+        Timed _t(c, "p4_ifft_y", (double)pp.ni * pp.nj * pp.nk2);
+        hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
+    }
     DYN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_ifft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kcx);
+    {
+        Timed _t(c, "p5_ifft_x_out", (double)pp.ni * pp.nj * pp.nk2);
+        hipLaunchKernelGGL(k_ifft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kcx);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }

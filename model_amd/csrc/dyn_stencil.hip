@@ -504,10 +504,13 @@ int advect_uvw_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
     dim3 blk, grd = col_grid(c->geo, c->geo.NZ - 2, jl, jh, il, ih, blk);
     bool moist = c->g.microphysics != 0;
     auto P = [&](int f, int r) { return (T *)c->prog[f][r]; };
-    hipLaunchKernelGGL(k_advect_uvw<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_U, 1), P(DYN_V, 1), P(DYN_W, 1),
+    {
+        Timed _t(c, "advect_uvw", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
+        hipLaunchKernelGGL(k_advect_uvw<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_U, 1), P(DYN_V, 1), P(DYN_W, 1),
                        (const T *)c->bar[0], (const T *)c->bar[1], (const T *)c->bar[2], P(DYN_U, 0), P(DYN_V, 0), P(DYN_W, 0),
                        P(DYN_TH, 1), moist ? P(DYN_QV, 1) : nullptr, moist ? P(DYN_QC, 1) : nullptr, moist ? P(DYN_QR, 1) : nullptr,
                        P(DYN_U, 2), P(DYN_V, 2), P(DYN_W, 2), T(step), il, jl, jh, moist ? 1 : 0);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -516,9 +519,12 @@ template <typename T>
 int advect_theta_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
     dim3 blk, grd = col_grid(c->geo, c->geo.NZ - 2, jl, jh, il, ih, blk);
     auto P = [&](int f, int r) { return (T *)c->prog[f][r]; };
-    hipLaunchKernelGGL(k_advect_theta<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_U, 1), P(DYN_V, 1), P(DYN_W, 1),
+    {
+        Timed _t(c, "advect_theta", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
+        hipLaunchKernelGGL(k_advect_theta<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_U, 1), P(DYN_V, 1), P(DYN_W, 1),
                        (const T *)c->bar[0], (const T *)c->bar[1], (const T *)c->bar[2], P(DYN_TH, 1), (const T *)c->bar[3],
                        P(DYN_TH, 0), P(DYN_TH, 2), T(step), il, jl, jh);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -527,13 +533,19 @@ template <typename T>
 int advect_moisture_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
     dim3 blk, grd = col_grid(c->geo, c->geo.NZ - 2, jl, jh, il, ih, blk);
     auto P = [&](int f, int r) { return (T *)c->prog[f][r]; };
-    hipLaunchKernelGGL(k_advect_moisture<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_U, 1), P(DYN_V, 1), P(DYN_W, 1),
+    {
+        Timed _t(c, "advect_moisture", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
+        hipLaunchKernelGGL(k_advect_moisture<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_U, 1), P(DYN_V, 1), P(DYN_W, 1),
                        (const T *)c->bar[0], (const T *)c->bar[1], (const T *)c->bar[2], P(DYN_QV, 1), P(DYN_QC, 1), P(DYN_QR, 1),
                        (const T *)c->bar[4], P(DYN_QV, 0), P(DYN_QC, 0), P(DYN_QR, 0), P(DYN_QV, 2), P(DYN_QC, 2), P(DYN_QR, 2),
                        (T *)c->vt, T(step), il, jl, jh);
+    }
     DYN_LAUNCH_CHECK();
     // the stored fall speed is written after the advection kernel has read the old level NZ-1 values
-    hipLaunchKernelGGL(k_fall_speed<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_QR, 1), (T *)c->vt, il, jl, jh);
+    {
+        Timed _t(c, "fall_speed", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
+        hipLaunchKernelGGL(k_fall_speed<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_QR, 1), (T *)c->vt, il, jl, jh);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -541,8 +553,11 @@ int advect_moisture_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
 template <typename T>
 int zero_moisture_t(Ctx *c, int il, int ih, int jl, int jh) {
     long long n = (long long)c->npts;
-    hipLaunchKernelGGL(k_zero_moisture<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->geo, c->cols,
+    {
+        Timed _t(c, "zero_moisture", (double)n);
+        hipLaunchKernelGGL(k_zero_moisture<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->geo, c->cols,
                        (T *)c->prog[DYN_QV][2], (T *)c->prog[DYN_QC][2], (T *)c->prog[DYN_QR][2], (const T *)c->bar[4], n, il, ih, jl, jh);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -550,8 +565,11 @@ int zero_moisture_t(Ctx *c, int il, int ih, int jl, int jh) {
 template <typename T>
 int pgf_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
     dim3 blk, grd = col_grid(c->geo, c->geo.NZ - 2, jl, jh, il, ih, blk);
-    hipLaunchKernelGGL(k_pgf<T>, grd, blk, 0, c->stream, c->geo, c->cols, (const T *)c->pi, (T *)c->prog[DYN_U][2],
+    {
+        Timed _t(c, "pgf", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
+        hipLaunchKernelGGL(k_pgf<T>, grd, blk, 0, c->stream, c->geo, c->cols, (const T *)c->pi, (T *)c->prog[DYN_U][2],
                        (T *)c->prog[DYN_V][2], (T *)c->prog[DYN_W][2], T(step), il, jl, jh);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -562,7 +580,10 @@ int periodic_t(Ctx *c, void *const *fields, int nf) {
     fl.n = nf;
     for (int f = 0; f < nf; f++) fl.f[f] = fields[f];
     long long n = c->geo.NYNZ;
-    hipLaunchKernelGGL(k_periodic<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->geo, fl);
+    {
+        Timed _t(c, "periodic", 6.0 * n);
+        hipLaunchKernelGGL(k_periodic<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->geo, fl);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -587,14 +608,23 @@ int bc_t(Ctx *c, bool micro) {
     }
     int rows = c->g.sponge_south + c->g.sponge_north;
     if (rows > 0) {
+        {
+        Timed _t(c, "bc_sponge", (double)rows * (g.NX - 1) * (g.NZ - 1));
         hipLaunchKernelGGL(k_sponge_ns<T>, dim3((g.NZ - 1 + 63) / 64, rows, g.NX - 1), dim3(64), 0, c->stream, g, sp, c->sponge_s,
                            c->sponge_n, c->g.sponge_south, c->g.sponge_north);
+    }
         DYN_LAUNCH_CHECK();
     }
     long long n = g.NYNZ;
-    hipLaunchKernelGGL(k_periodic<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, per);
+    {
+        Timed _t(c, "bc_periodic", 6.0 * n);
+        hipLaunchKernelGGL(k_periodic<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, per);
+    }
     DYN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_upper_lower<T>, dim3((g.NY - 1 + 63) / 64, g.NX - 1), dim3(64), 0, c->stream, g, ul, nfill);
+    {
+        Timed _t(c, "bc_upper_lower", 2.0 * (g.NX - 1) * (g.NY - 1));
+        hipLaunchKernelGGL(k_upper_lower<T>, dim3((g.NY - 1 + 63) / 64, g.NX - 1), dim3(64), 0, c->stream, g, ul, nfill);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -603,11 +633,17 @@ template <typename T>
 int kessler_t(Ctx *c, int il, int ih, int jl, int jh) {
     dim3 blk, grd = col_grid(c->geo, c->geo.NZ - 2, jl, jh, il, ih, blk);
     auto P = [&](int f) { return (T *)c->prog[f][2]; };
-    hipLaunchKernelGGL(k_kessler<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_TH), P(DYN_QV), P(DYN_QC), P(DYN_QR),
+    {
+        Timed _t(c, "kessler", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
+        hipLaunchKernelGGL(k_kessler<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_TH), P(DYN_QV), P(DYN_QC), P(DYN_QR),
                        (const T *)c->bar[3], (const T *)c->bar[4], (const T *)c->bar[5], (const T *)c->pi, c->dflag, il, jl, jh);
+    }
     DYN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_precip<T>, dim3((jh - jl + 63) / 64, ih - il), dim3(64), 0, c->stream, c->geo, c->cols, (const T *)c->vt,
+    {
+        Timed _t(c, "precip", (double)(ih - il) * (jh - jl));
+        hipLaunchKernelGGL(k_precip<T>, dim3((jh - jl + 63) / 64, ih - il), dim3(64), 0, c->stream, c->geo, c->cols, (const T *)c->vt,
                        P(DYN_QR), (T *)c->accrain, il, ih, jl, jh);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -616,8 +652,11 @@ template <typename T>
 int rayleigh_t(Ctx *c, int il, int ih, int jl, int jh, int level) {
     dim3 blk(64, 4, 1), grd((c->geo.NZ - level + 63) / 64, (jh - jl + 3) / 4, ih - il);
     auto P = [&](int f) { return (T *)c->prog[f][2]; };
-    hipLaunchKernelGGL(k_rayleigh<T>, grd, blk, 0, c->stream, c->geo, c->raycoef, P(DYN_U), P(DYN_V), P(DYN_W), P(DYN_TH), il, jl, jh,
+    {
+        Timed _t(c, "rayleigh", (double)(ih - il) * (jh - jl) * (c->geo.NZ - level));
+        hipLaunchKernelGGL(k_rayleigh<T>, grd, blk, 0, c->stream, c->geo, c->raycoef, P(DYN_U), P(DYN_V), P(DYN_W), P(DYN_TH), il, jl, jh,
                        level);
+    }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
