@@ -30,7 +30,12 @@ PEAK_HBM_GBS = 8000.0
 PASSES = {
     "advect_uvw": {0: 13, 1: 16}, "advect_theta": 10, "advect_moisture": 16, "fall_speed": 2, "zero_moisture": 7,
     "pgf": 7, "p1_div_fft_x": 4, "p2_fft_y": 2, "p3_thomas": 2, "p4_ifft_y": 2, "p5_ifft_x_out": 2, "kessler": 12,
-    "rayleigh": 8, "advance_copy": 2, "fused_stage": None,
+    "rayleigh": 8, "advance_copy": 2,
+    # fused FP32 stage: current U V W TH QV QC QR, basic UB VB WB THB QB, M buffers (7) read; 7 forecasts
+    # and the fall speed written
+    "fused_advect": {0: 8 + 4 + 4 + 4, 1: 7 + 5 + 7 + 8},
+    "p1_div_rfft_x": 4, "p2_ypass": 2, "p5_irfft_x_out": 2,
+    "post_uvw": 7, "post_scalars": 1, "zero_moisture_shell": 1,
 }
 
 
@@ -50,9 +55,10 @@ def time_form(case, precision, steps, warmup):
     return g, (t1 - t0) * 1e3 / steps
 
 
-def kernel_profile(g, steps):
+def kernel_profile(g, steps, fused):
+    """Per-kernel HIP-event timings of the same step sequence the graph replays, launched eagerly."""
     g.timing(1)
-    g.step_reference(steps)
+    (g.step_fused if fused else g.step_reference)(steps)
     g.sync()
     import ctypes
     buf = ctypes.create_string_buffer(1 << 16)
@@ -129,7 +135,7 @@ def main():
         log(f"bench: {name} {ms:.3f} ms/step, {res[name]['value']:.4g} gridpoint-steps/s")
     main_form = forms[0][0]
     g = handles[main_form]
-    rows = kernel_profile(g, args.profile_steps)
+    rows = kernel_profile(g, args.profile_steps, fused=(main_form == "fp32"))
     for r in sorted(rows, key=lambda r: -r[2]):
         log(f"bench: kernel {r[0]:16s} launches {r[1]:5d} total {r[2]:9.3f} ms avg {r[2] / r[1] * 1e3:8.1f} us")
     roof = roofline(rows, 4 if main_form == "fp32" else 8, case.microphysics)

@@ -102,8 +102,12 @@ int dyn_micro_advance_outer(dyn_ctx *ctx);
 /* ---- whole steps ------------------------------------------------------------------------- */
 /* s_integrate_rk3 (driver_serial.cpp:178-286) as the sequence of per-operator calls above */
 int dyn_step_reference(dyn_ctx *ctx, int nsteps);
-/* the same step with fused stage kernels, replayed from a captured hipGraph */
+/* the step replayed from a captured hipGraph: for DYN_FP32 contexts the stage advection runs as one
+ * fused kernel (same values as the per-operator kernels, bit for bit); DYN_FP64 contexts replay the
+ * per-operator sequence */
 int dyn_step(dyn_ctx *ctx, int nsteps);
+/* the fused-stage step launched eagerly (either precision; needs NZ <= 64) */
+int dyn_step_fused(dyn_ctx *ctx, int nsteps);
 
 /* ---- measurement ------------------------------------------------------------------------- */
 /* HBM bytes the step's algorithm must move per gridpoint (the field-pass count of the operator set) */

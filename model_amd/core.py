@@ -25,7 +25,7 @@ EXPORTS = [
     "dyn_synchronize", "dyn_advect_uvw", "dyn_advect_theta", "dyn_advect_moisture", "dyn_zero_moisture",
     "dyn_solve_pressure", "dyn_pgf", "dyn_apply_bc", "dyn_apply_bc_micro", "dyn_kessler", "dyn_rayleigh",
     "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner", "dyn_micro_advance_outer",
-    "dyn_step_reference", "dyn_step", "dyn_algorithmic_bytes_per_point", "dyn_timing", "dyn_timing_report",
+    "dyn_step_reference", "dyn_step", "dyn_step_fused", "dyn_algorithmic_bytes_per_point", "dyn_timing", "dyn_timing_report",
     "dyn_stream",
 ]
 
@@ -73,6 +73,7 @@ def load():
         getattr(L, fn).argtypes = [ctypes.c_void_p]
     L.dyn_step_reference.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dyn_step_fused.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_algorithmic_bytes_per_point.argtypes = [ctypes.c_void_p]
     L.dyn_algorithmic_bytes_per_point.restype = ctypes.c_double
     L.dyn_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -150,6 +151,9 @@ class DynCore:
 
     def step_reference(self, n=1):
         _check(self.L.dyn_step_reference(self.h, n))
+
+    def step_fused(self, n=1):
+        _check(self.L.dyn_step_fused(self.h, n))
 
     @property
     def stream(self):

@@ -39,6 +39,11 @@ Timed::~Timed() {
 }
 
 int pressure_coefficients(Ctx *c);
+bool fused_supported(const Ctx *c);
+int fused_prepare(Ctx *c);
+int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh);
+int launch_zero_moisture_shell(Ctx *c, int il, int ih, int jl, int jh);
+int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip);
 
 namespace {
 
@@ -225,6 +230,10 @@ int dyn_create(const dyn_grid *grid, dyn_ctx **out) {
     if (hipMalloc(&c->dflag, sizeof(int)) != hipSuccess || hipMemset(c->dflag, 0, sizeof(int)) != hipSuccess) return fail(DYN_E_HIP);
     int rc = pressure_init(c);
     if (rc) return fail(rc);
+    if (fused_supported(c)) {
+        rc = fused_prepare(c);
+        if (rc) return fail(rc);
+    }
     if (hipDeviceSynchronize() != hipSuccess) return fail(DYN_E_HIP);
     *out = reinterpret_cast<dyn_ctx *>(c);
     return DYN_OK;
@@ -560,6 +569,68 @@ static int reference_step(dyn_ctx *ctx) {
     return DYN_OK;
 }
 
+// The same step with the stage advection fused into one kernel (advect_uvw_velocity, advect_theta,
+// advect_microphysics_cell and zero_moisture's interior in one pass).  advect_theta and the moisture
+// advection read only the current-stage fields, which the pressure solve and the gradient force never
+// write, so they move ahead of solve_pressure without changing any value.
+static int fused_step(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    const double steps[] = {1. / 3., 0.5, 1.0};
+    const int NX = c->g.nx, NY = c->g.ny;
+    const bool moist = c->g.microphysics != 0;
+    int rc;
+#define RC(x)                      \
+    do {                           \
+        if ((rc = (x))) return rc; \
+    } while (0)
+    for (int s = 0; s < 3; s++) {
+        double step = steps[s];
+        RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3));
+        // the halo columns periodic_uvw_boundaries writes are read only by the divergence (which reads
+        // UP(3) for UP(NX-3)); the boundary update overwrites them before anything else reads them
+        RC(launch_solve_pressure(c, step, false));
+        // pressure gradient, zero_moisture outside the advected box and both boundary updates
+        RC(launch_post_stage(c, step, true, true, true));
+        if (moist && s < 2) RC(dyn_micro_advance_inner(ctx));
+        if (s < 2) RC(dyn_advance_inner(ctx));
+    }
+    if (moist) RC(dyn_kessler(ctx, 1, NX - 1, 1, NY - 1));
+    // apply_boundary_condition_microphysics then apply_boundary_condition (different fields, so one
+    // kernel serves both; the moisture pointers swap only afterwards)
+    RC(launch_post_stage(c, 0.0, false, true, false));
+    if (moist) RC(dyn_micro_advance_outer(ctx));
+    RC(dyn_rayleigh(ctx, 1, NX - 1, 1, NY - 1, c->g.rayleigh_level));
+    RC(dyn_advance_outer(ctx));
+#undef RC
+    return DYN_OK;
+}
+
+// the sequence dyn_step records: fused stages for the FP32 form, per-operator kernels for FP64
+static int graph_step(dyn_ctx *ctx) {
+    Ctx *c = as_ctx(ctx);
+    if (c->prec == DYN_FP32 && fused_supported(c)) return fused_step(ctx);
+    return reference_step(ctx);
+}
+
+int dyn_step_fused(dyn_ctx *ctx, int nsteps) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    CHECK_COLS(c);
+    if (!fused_supported(c)) {
+        set_error("dyn_step_fused: the fused stage kernel needs NZ <= 64");
+        return DYN_E_UNSUPPORTED;
+    }
+    if (c->g.rayleigh_level < c->g.nz) {
+        int rc = ensure_rayleigh_table(c, c->g.rayleigh_level);
+        if (rc) return rc;
+    }
+    for (int n = 0; n < nsteps; n++) {
+        int rc = fused_step(ctx);
+        if (rc) return rc;
+    }
+    return DYN_OK;
+}
+
 int dyn_step_reference(dyn_ctx *ctx, int nsteps) {
     Ctx *c = as_ctx(ctx);
     CHECK_CTX(c);
@@ -590,8 +661,8 @@ int dyn_step(dyn_ctx *ctx, int nsteps) {
             memcpy(saved, c->prog, sizeof(saved));
             DYN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             c->capturing = true;
-            int rc = reference_step(ctx);
-            if (!rc) rc = reference_step(ctx);
+            int rc = graph_step(ctx);
+            if (!rc) rc = graph_step(ctx);
             c->capturing = false;
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &graph);
@@ -614,7 +685,7 @@ int dyn_step(dyn_ctx *ctx, int nsteps) {
         for (int p = 0; p < pairs; p++) DYN_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     }
     if (nsteps % 2) {
-        int rc = reference_step(ctx);
+        int rc = graph_step(ctx);
         if (rc) return rc;
     }
     return DYN_OK;

@@ -62,7 +62,9 @@ struct PressurePlan {
     double *coskx = nullptr, *cosly = nullptr; // cos(kwave[i]), cos(lwave[j]) as init_fft3d evaluates them
     double *acoef = nullptr, *ccoef = nullptr; // pressure.cpp:779-782
     void *spec = nullptr;              // spectral work array [kx][ky][kk] complex (T)
-    double *ffac = nullptr;            // Thomas multipliers F2 [kx][ky][kk] (FP64)
+    double *ffac = nullptr;            // Thomas multipliers F2 [kx][kk][ky] (FP64)
+    bool fused_y = false;              // j-transforms and the vertical solve in one LDS-resident kernel
+    size_t lds_max = 0;
 };
 
 // per-kernel HIP-event timing on the context stream (eager launches only)
@@ -130,7 +132,7 @@ int launch_kessler(Ctx *c, int il, int ih, int jl, int jh);
 int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level);
 int pressure_init(Ctx *c);
 int pressure_free(Ctx *c);
-int launch_solve_pressure(Ctx *c, double step);
+int launch_solve_pressure(Ctx *c, double step, bool halos = true);
 int launch_convert(Ctx *c, void *dst, const double *src, size_t n, bool to_device_type);
 
 }  // namespace dyn

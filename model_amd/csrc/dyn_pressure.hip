@@ -233,7 +233,9 @@ __global__ void k_thomas_factors(int ni, int nj, int nk, const double *__restric
     if (q >= ni * nj) return;
     int kx = q / nj, ky = q % nj;
     int nk2 = nk - 2;
-    double *F = ffac + (long long)q * nk2;
+    // layout [kx][kk][ky]: the columns of one kx are adjacent for every level
+    double *F = ffac + (long long)kx * nk2 * nj + ky;
+    const long long fs = nj;
     double ck = coskx[kx], cl = cosly[ky];
     // B2 of pressure.cpp:787-791
     double B = rhou[1] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[1] * mw[2] * rhow[2] / (dz * dz);
@@ -245,7 +247,7 @@ __global__ void k_thomas_factors(int ni, int nj, int nk, const double *__restric
         else B = rhou[nk - 2] * 2.0 * (ck + cl - 2.0) / (dx * dx) - (mu[nk - 2] * mw[nk - 2] * rhow[nk - 2]) / (dz * dz);
         z = 1.0 / (B - acoef[k] * Fp);
         Fp = ccoef[k] * z;
-        F[k - 1] = Fp;
+        F[(k - 1) * fs] = Fp;
     }
 }
 
@@ -261,7 +263,8 @@ __global__ void k_thomas(T *__restrict__ spec, int ni, int nj, int nk, const dou
     int kx = q / nj, ky = q % nj;
     const int nk2 = nk - 2;
     T *x = spec + (long long)q * nk2 * 2;
-    const double *F = ffac + (long long)q * nk2;
+    const long long fs = nj;
+    const double *F = ffac + (long long)kx * nk2 * nj + ky;
     const double size2d = (double)(ni * nj);
     double ck = coskx[kx], cl = cosly[ky];
     double B = rhou[1] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[1] * mw[2] * rhow[2] / (dz * dz);
@@ -269,8 +272,6 @@ __global__ void k_thomas(T *__restrict__ spec, int ni, int nj, int nk, const dou
     double pr = (double(x[0]) / size2d) * z, pi = (double(x[1]) / size2d) * z;
     x[0] = T(pr);
     x[1] = T(pi);
-    // keep the eliminated values in FP64 in a second pass-through: store FP64 in a local ring is
-    // not possible for arbitrary nk, so FP32 storage rounds here (the FP64 path is exact).
     double Fp = F[0];
     for (int k = 2; k < nk - 1; k++) {
         if (k < nk - 2) B = rhou[k] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[k] * (mw[k + 1] * rhow[k + 1] + mw[k] * rhow[k]) / (dz * dz);
@@ -281,7 +282,7 @@ __global__ void k_thomas(T *__restrict__ spec, int ni, int nj, int nk, const dou
         pi = (xi - acoef[k] * pi) * z;
         x[2 * (k - 1)] = T(pr);
         x[2 * (k - 1) + 1] = T(pi);
-        Fp = F[k - 1];
+        Fp = F[(k - 1) * fs];
     }
     if (kx == 0 && ky == 0) {  // OUT3D(0,0,NZ-2) = 0
         pr = 0.0;
@@ -290,7 +291,7 @@ __global__ void k_thomas(T *__restrict__ spec, int ni, int nj, int nk, const dou
         x[2 * (nk2 - 1) + 1] = T(0);
     }
     for (int k = nk - 3; k >= 1; k--) {
-        double f = F[k - 1];
+        double f = F[(k - 1) * fs];
         pr = double(x[2 * (k - 1)]) - f * pr;
         pi = double(x[2 * (k - 1) + 1]) - f * pi;
         x[2 * (k - 1)] = T(pr);
@@ -339,6 +340,293 @@ __global__ void __launch_bounds__(kThreads) k_ifft_x_out(Geo g, const T *__restr
     }
 }
 
+
+// ---- half-spectrum path (even ni) ------------------------------------------------------------------
+// The divergence is real, so two levels travel as the real and imaginary parts of one complex DFT along
+// i and only kx = 0..ni/2 is kept (the rest is the conjugate mirror).  After the j-transform and the
+// tridiagonal solve the real part of the inverse is rebuilt from that half, which is what taking the real
+// part of the full inverse (pressure.cpp:881-890) computes.
+
+// in-place Stockham DFT over LDS: element (n, b) at re[n*sn + b*sb], im[...]; every butterfly's inputs are
+// read into registers before a barrier, then written to their output slots
+template <typename T, int MAXB>
+__device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, int sb, const double *tw, int sign) {
+    const int n = rad.n;
+    const int nthr = blockDim.x;
+    int Ns = 1;
+    for (int s = 0; s < rad.nr; s++) {
+        const int R = rad.r[s];
+        const int nbf = n / R;
+        const int stride = n / (Ns * R);
+        const int total = nbf * nbatch;
+        T xr[MAXB][5], xi[MAXB][5];
+        int oidx[MAXB];
+#pragma unroll
+        for (int t = 0; t < MAXB; t++) {
+            int idx = threadIdx.x + t * nthr;
+            if (idx < total) {
+                int b = idx % nbatch, jj = idx / nbatch;
+                int jm = jj % Ns;
+#pragma unroll
+                for (int r = 0; r < 5; r++) {
+                    if (r < R) {
+                        int e = (jj + r * nbf) * sn + b * sb;
+                        xr[t][r] = re[e];
+                        xi[t][r] = im[e];
+                    }
+                }
+                if (Ns > 1) {
+#pragma unroll
+                    for (int r = 1; r < 5; r++) {
+                        if (r < R) {
+                            T wr, wi;
+                            twiddle(tw, r * jm * stride, sign, wr, wi);
+                            cmul(xr[t][r], xi[t][r], wr, wi);
+                        }
+                    }
+                }
+                butterfly(R, xr[t], xi[t], sign);
+                oidx[t] = ((jj / Ns) * Ns * R + jm) * sn + b * sb;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < MAXB; t++) {
+            int idx = threadIdx.x + t * nthr;
+            if (idx < total) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) {
+                    if (r < R) {
+                        re[oidx[t] + r * Ns * sn] = xr[t][r];
+                        im[oidx[t] + r * Ns * sn] = xi[t][r];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}
+
+constexpr int kMaxBX = 4;   // butterflies per thread in the x passes (register-resident)
+constexpr int kMaxBY = 4;   // and in the fused y pass
+
+// Q1: divergence3d_periodic prologue + forward real DFT along i; block = (j, chunk of kc levels, kc even).
+// The periodic column UP(NX-3) is read as UP(3), the value periodic_uvw_boundaries puts there.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T *__restrict__ UP, const T *__restrict__ VP,
+                                                         const T *__restrict__ WP, T *__restrict__ spec, Radix rad, const double *tw,
+                                                         int nk2, int kc, double step) {
+    extern __shared__ unsigned char smem[];
+    const int n = rad.n, P = kc / 2, nkx = n / 2 + 1;
+    T *re = (T *)smem, *im = re + n * P;
+    const int j = blockIdx.x;
+    const int kk0 = blockIdx.y * kc;
+    const int jj = j == 0 ? 1 : (j == g.NY - 1 ? g.NY - 2 : j);  // mirrored rows, pressure.cpp:1408-1415
+    const double sdt = step * g.dt;
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, i = idx / kc;
+        int kk = kk0 + b;
+        T d = T(0);
+        if (kk < nk2) {
+            int k = kk + 1, ia = i + 3;
+            int ie = ia + 1 == g.NX - 3 ? 3 : ia + 1;
+            T rhou = T(c.rhou[k]);
+            d = rhou * (UP[IX(ie, jj, k)] - UP[IX(ia, jj, k)]) * T(g.one_d_dx) +
+                rhou * (VP[IX(ia, jj + 1, k)] - VP[IX(ia, jj, k)]) * T(g.one_d_dy) +
+                (T(c.rhow[k + 1]) * WP[IX(ia, jj, k + 1)] - T(c.rhow[k]) * WP[IX(ia, jj, k)]) * (T(g.one_d_dz) * T(c.mu[k]));
+            d /= T(sdt);
+        }
+        if (b & 1) im[i * P + (b >> 1)] = d;
+        else re[i * P + (b >> 1)] = d;
+    }
+    __syncthreads();
+    lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, tw, -1);
+    // split the packed pair: A = (Z[m] + conj Z[-m]) / 2, B = (Z[m] - conj Z[-m]) / 2i
+    for (int idx = threadIdx.x; idx < nkx * P; idx += blockDim.x) {
+        int pp = idx % P, m = idx / P;
+        int mm = m == 0 ? 0 : n - m;
+        T zr = re[m * P + pp], zi = im[m * P + pp], wr = re[mm * P + pp], wi = im[mm * P + pp];
+        int ka = kk0 + 2 * pp, kb = ka + 1;
+        long long o = ((long long)m * g.NY + j) * nk2;
+        if (ka < nk2) {
+            spec[(o + ka) * 2] = T(0.5) * (zr + wr);
+            spec[(o + ka) * 2 + 1] = T(0.5) * (zi - wi);
+        }
+        if (kb < nk2) {
+            spec[(o + kb) * 2] = T(0.5) * (zi + wi);
+            spec[(o + kb) * 2 + 1] = T(0.5) * (wr - zr);
+        }
+    }
+}
+
+// Q3: backward real DFT along i -> PI (pressure.cpp:878-892) with the ghost levels and periodic halos
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_irfft_x_out(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
+                                                          const double *tw, int nk2, int kc) {
+    extern __shared__ unsigned char smem[];
+    const int n = rad.n, P = kc / 2, half = n / 2;
+    T *re = (T *)smem, *im = re + n * P;
+    const int j = blockIdx.x, kk0 = blockIdx.y * kc;
+    // rebuild Z = A + iB for m in [0, n) from the half spectra of the two levels of each pair
+    for (int idx = threadIdx.x; idx < n * P; idx += blockDim.x) {
+        int pp = idx % P, m = idx / P;
+        int ka = kk0 + 2 * pp, kb = ka + 1;
+        int ms = m <= half ? m : n - m;
+        long long o = ((long long)ms * g.NY + j) * nk2;
+        T ar = T(0), ai = T(0), br = T(0), bi = T(0);
+        if (ka < nk2) {
+            ar = spec[(o + ka) * 2];
+            ai = spec[(o + ka) * 2 + 1];
+        }
+        if (kb < nk2) {
+            br = spec[(o + kb) * 2];
+            bi = spec[(o + kb) * 2 + 1];
+        }
+        if (ms == 0 || ms == half) {  // self-conjugate wavenumbers: only the real part is kept
+            ai = T(0);
+            bi = T(0);
+        }
+        if (m > half) {  // conjugate mirror
+            ai = -ai;
+            bi = -bi;
+        }
+        re[m * P + pp] = ar - bi;
+        im[m * P + pp] = ai + br;
+    }
+    __syncthreads();
+    lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, tw, +1);
+    const int shift = g.NX - 6;
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        int b = idx % kc, i = idx / kc, kk = kk0 + b;
+        if (kk >= nk2) continue;
+        T v = (b & 1) ? im[i * P + (b >> 1)] : re[i * P + (b >> 1)];
+        int ia = i + 3, k = kk + 1;
+        int ks[3] = {k, -1, -1};
+        if (kk == 0) ks[1] = 0;
+        if (kk == nk2 - 1) ks[2] = g.NZ - 1;
+        for (int t = 0; t < 3; t++) {
+            if (ks[t] < 0) continue;
+            PI[IX(ia, j, ks[t])] = v;
+            if (ia >= shift && ia < shift + 3) PI[IX(ia - shift, j, ks[t])] = v;
+            if (ia >= 3 && ia < 6) PI[IX(ia + shift, j, ks[t])] = v;
+        }
+    }
+}
+
+// B2 of pressure.cpp:787-791 for level k (1..nk-2) of column (ck, cl)
+__device__ __forceinline__ double bcoef(int k, int nk, double ck, double cl, const double *rhou, const double *rhow, const double *mu,
+                                        const double *mw, double dx, double dz) {
+    if (k == 1) return rhou[1] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[1] * mw[2] * rhow[2] / (dz * dz);
+    if (k < nk - 2) return rhou[k] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[k] * (mw[k + 1] * rhow[k + 1] + mw[k] * rhow[k]) / (dz * dz);
+    return rhou[nk - 2] * 2.0 * (ck + cl - 2.0) / (dx * dx) - (mu[nk - 2] * mw[nk - 2] * rhow[nk - 2]) / (dz * dz);
+}
+
+struct ThomasArgs {
+    const double *coskx, *cosly, *rhou, *rhow, *mu, *mw, *acoef, *ffac;
+    double dx, dz;
+    int ni, nj, nk;
+};
+
+// Forward elimination and back substitution of one (kx,ky) column whose values sit at x[kk*xs] (re) and
+// x[kk*xs + is] (im); FP64 arithmetic with the multipliers from the table; the (0,0) column keeps its
+// eliminated values in the FP64 scratch `mean` instead of the storage type (pressure.cpp:837-871).
+template <typename T>
+__device__ void thomas_column(const ThomasArgs &ta, int kx, int ky, T *x, int xs, int is, double *mean) {
+    const int nk = ta.nk, nk2 = nk - 2;
+    const long long fs = ta.nj;
+    const double *F = ta.ffac + (long long)kx * nk2 * ta.nj + ky;
+    const double size2d = (double)(ta.ni * ta.nj);
+    const double ck = ta.coskx[kx], cl = ta.cosly[ky];
+    const bool m0 = (kx == 0 && ky == 0);
+    double z = 1.0 / bcoef(1, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz);
+    double pr = (double(x[0]) / size2d) * z, pi = (double(x[is]) / size2d) * z;
+    if (m0) { mean[0] = pr; mean[1] = pi; }
+    else { x[0] = T(pr); x[is] = T(pi); }
+    double Fp = F[0];
+    for (int k = 2; k < nk - 1; k++) {
+        z = 1.0 / (bcoef(k, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz) - ta.acoef[k] * Fp);
+        int o = (k - 1) * xs;
+        double xr = double(x[o]) / size2d, xi = double(x[o + is]) / size2d;
+        pr = (xr - ta.acoef[k] * pr) * z;
+        pi = (xi - ta.acoef[k] * pi) * z;
+        if (m0) { mean[2 * (k - 1)] = pr; mean[2 * (k - 1) + 1] = pi; }
+        else { x[o] = T(pr); x[o + is] = T(pi); }
+        Fp = F[(k - 1) * fs];
+    }
+    if (m0) {  // OUT3D(0,0,NZ-2) = 0
+        pr = 0.0;
+        pi = 0.0;
+    }
+    x[(nk2 - 1) * xs] = T(pr);
+    x[(nk2 - 1) * xs + is] = T(pi);
+    for (int k = nk - 3; k >= 1; k--) {
+        int o = (k - 1) * xs;
+        double f = F[(k - 1) * fs];
+        double yr = m0 ? mean[2 * (k - 1)] : double(x[o]);
+        double yi = m0 ? mean[2 * (k - 1) + 1] : double(x[o + is]);
+        pr = yr - f * pr;
+        pi = yi - f * pi;
+        x[o] = T(pr);
+        x[o + is] = T(pi);
+    }
+}
+
+// Q2 (fused): forward DFT along j, FP64 tridiagonal solve, backward DFT along j for one kx, with the
+// whole (j, k) slab of that kx resident in LDS
+template <typename T>
+__global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, Radix rady, const double *twy, ThomasArgs ta,
+                                               int pad) {
+    extern __shared__ unsigned char smem[];
+    const int nj = rady.n, P2 = nk2 + pad;
+    T *re = (T *)smem, *im = re + nj * P2;
+    double *mean = (double *)(re + 2 * nj * P2);  // 2*nj*P2 elements of T: 8-byte aligned
+    const int kx = blockIdx.x;
+    const long long base = (long long)kx * nj * nk2 * 2;
+    for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
+        int kk = idx % nk2, ky = idx / nk2;
+        re[ky * P2 + kk] = spec[base + 2 * (long long)idx];
+        im[ky * P2 + kk] = spec[base + 2 * (long long)idx + 1];
+    }
+    __syncthreads();
+    lds_fft_ip<T, kMaxBY>(re, im, rady, nk2, P2, 1, twy, -1);
+    if (threadIdx.x < nj) {
+        int ky = threadIdx.x;
+        // re and im of one column sit in separate planes: x[kk] = re[ky*P2 + kk], x[kk + is] = im[...]
+        thomas_column<T>(ta, kx, ky, re + ky * P2, 1, (int)(im - re), mean);
+    }
+    __syncthreads();
+    lds_fft_ip<T, kMaxBY>(re, im, rady, nk2, P2, 1, twy, +1);
+    for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
+        int kk = idx % nk2, ky = idx / nk2;
+        spec[base + 2 * (long long)idx] = re[ky * P2 + kk];
+        spec[base + 2 * (long long)idx + 1] = im[ky * P2 + kk];
+    }
+}
+
+// Q2 (staged): tridiagonal solve of tg columns of one kx staged through LDS (coalesced in and out)
+template <typename T>
+__global__ void __launch_bounds__(64) k_thomas_staged(T *__restrict__ spec, int nk2, ThomasArgs ta) {
+    extern __shared__ unsigned char smem[];
+    const int tg = blockDim.x, P2 = 2 * nk2 + 1;
+    T *x = (T *)smem;
+    double *mean = (double *)(smem + ((tg * P2 * sizeof(T) + 7) / 8) * 8);
+    const int kx = blockIdx.x, ky0 = blockIdx.y * tg;
+    const int ncol = min(tg, ta.nj - ky0);
+    const long long base = ((long long)kx * ta.nj + ky0) * nk2 * 2;
+    for (int idx = threadIdx.x; idx < ncol * nk2 * 2; idx += blockDim.x) {
+        int col = idx / (2 * nk2), rem = idx % (2 * nk2);
+        x[col * P2 + rem] = spec[base + idx];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncol) thomas_column<T>(ta, kx, ky0 + threadIdx.x, x + threadIdx.x * P2, 2, 1, mean);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < ncol * nk2 * 2; idx += blockDim.x) {
+        int col = idx / (2 * nk2), rem = idx % (2 * nk2);
+        spec[base + idx] = x[col * P2 + rem];
+    }
+}
+
 bool factorize(int n, Radix &rad) {
     rad.n = n;
     rad.nr = 0;
@@ -374,46 +662,107 @@ double *upload_twiddles(int n, hipStream_t s) {
     return d;
 }
 
+int max_butterflies(const Radix &r, int nbatch, int nthr) {
+    int mb = 0;
+    for (int s = 0; s < r.nr; s++) mb = std::max(mb, (r.n / r.r[s] * nbatch + nthr - 1) / nthr);
+    return mb;
+}
+
 template <typename T>
-int solve_t(Ctx *c, double step) {
+int solve_t(Ctx *c, double step, bool halos) {
     PressurePlan &pp = c->pp;
     const Geo &g = c->geo;
     Radix rx{}, ry{};
     factorize(pp.ni, rx);
     factorize(pp.nj, ry);
-    // periodic_uvw_boundaries (boundaries.cpp:546-555)
-    void *uvw[3] = {c->prog[DYN_U][2], c->prog[DYN_V][2], c->prog[DYN_W][2]};
-    int rc = launch_periodic(c, uvw, 3);
-    if (rc) return rc;
+    if (halos) {  // periodic_uvw_boundaries (boundaries.cpp:546-555)
+        void *uvw[3] = {c->prog[DYN_U][2], c->prog[DYN_V][2], c->prog[DYN_W][2]};
+        int rc = launch_periodic(c, uvw, 3);
+        if (rc) return rc;
+    }
+    T *spec = (T *)pp.spec;
+    const double pts = (double)pp.ni * pp.nj * pp.nk2;
+    ThomasArgs ta{pp.coskx, pp.cosly, c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz,
+                  pp.ni, pp.nj, c->g.nz};
+    if (pp.ni % 2 == 0) {
+        const int nkx = pp.ni / 2 + 1;
+        // level pairs per block: enough blocks to fill the chip, the LDS tile within budget
+        int kc = (int)(kLdsBudget / (2 * (size_t)pp.ni * sizeof(T))) * 2;
+        int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
+        kc = std::max(2, std::min(kc, want));
+        while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
+        size_t shx = 2 * (size_t)pp.ni * (kc / 2) * sizeof(T);
+        dim3 gx(g.NY, (pp.nk2 + kc - 1) / kc);
+        {
+            Timed _t(c, "p1_div_rfft_x", pts);
+            hipLaunchKernelGGL(k_div_rfft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
+                               (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kc, step);
+        }
+        DYN_LAUNCH_CHECK();
+        const int pad = 1;
+        size_t shy = 2 * (size_t)pp.nj * (pp.nk2 + pad) * sizeof(T) + 8 + 16 * (size_t)pp.nk2 + 8;
+        if (pp.fused_y && shy <= pp.lds_max && max_butterflies(ry, pp.nk2, 1024) <= kMaxBY) {
+            Timed _t(c, "p2_ypass", pts);
+            hipLaunchKernelGGL(k_ypass<T>, dim3(nkx), dim3(1024), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
+            DYN_LAUNCH_CHECK();
+        } else {
+            int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
+            size_t sh2 = 4 * (size_t)pp.nj * kcy * sizeof(T);
+            dim3 gy(nkx, (pp.nk2 + kcy - 1) / kcy);
+            {
+                Timed _t(c, "p2_fft_y", pts);
+                hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
+            }
+            DYN_LAUNCH_CHECK();
+            const int tg = 64;
+            size_t sht = ((tg * (2 * (size_t)pp.nk2 + 1) * sizeof(T) + 7) / 8) * 8 + 16 * (size_t)pp.nk2;
+            {
+                Timed _t(c, "p3_thomas", pts);
+                hipLaunchKernelGGL(k_thomas_staged<T>, dim3(nkx, (pp.nj + tg - 1) / tg), dim3(tg), sht, c->stream, spec, pp.nk2, ta);
+            }
+            DYN_LAUNCH_CHECK();
+            {
+                Timed _t(c, "p4_ifft_y", pts);
+                hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
+            }
+            DYN_LAUNCH_CHECK();
+        }
+        {
+            Timed _t(c, "p5_irfft_x_out", pts);
+            hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kc);
+        }
+        DYN_LAUNCH_CHECK();
+        return DYN_OK;
+    }
+    // odd ni: full complex spectrum
     int kcx = level_chunk(pp.ni, sizeof(T), pp.nk2), kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
     size_t shx = 4 * (size_t)pp.ni * kcx * sizeof(T), shy = 4 * (size_t)pp.nj * kcy * sizeof(T);
     dim3 gx(g.NY, (pp.nk2 + kcx - 1) / kcx), gy(pp.ni, (pp.nk2 + kcy - 1) / kcy);
-    T *spec = (T *)pp.spec;
     {
-        Timed _t(c, "p1_div_fft_x", (double)pp.ni * pp.nj * pp.nk2);
+        Timed _t(c, "p1_div_fft_x", pts);
         hipLaunchKernelGGL(k_div_fft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
-                       (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kcx, step);
+                           (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kcx, step);
     }
     DYN_LAUNCH_CHECK();
     {
-        Timed _t(c, "p2_fft_y", (double)pp.ni * pp.nj * pp.nk2);
+        Timed _t(c, "p2_fft_y", pts);
         hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
     }
     DYN_LAUNCH_CHECK();
     int nq = pp.ni * pp.nj;
     {
-        Timed _t(c, "p3_thomas", (double)pp.ni * pp.nj * pp.nk2);
+        Timed _t(c, "p3_thomas", pts);
         hipLaunchKernelGGL(k_thomas<T>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, spec, pp.ni, pp.nj, g.NZ, pp.coskx, pp.cosly,
-                       c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz);
+                           c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz);
     }
     DYN_LAUNCH_CHECK();
     {
-        Timed _t(c, "p4_ifft_y", (double)pp.ni * pp.nj * pp.nk2);
+        Timed _t(c, "p4_ifft_y", pts);
         hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
     }
     DYN_LAUNCH_CHECK();
     {
-        Timed _t(c, "p5_ifft_x_out", (double)pp.ni * pp.nj * pp.nk2);
+        Timed _t(c, "p5_ifft_x_out", pts);
         hipLaunchKernelGGL(k_ifft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kcx);
     }
     DYN_LAUNCH_CHECK();
@@ -441,6 +790,17 @@ int pressure_init(Ctx *c) {
         return DYN_E_HIP;
     }
     DYN_HIP(hipMalloc(&pp.spec, sizeof(double) * 2 * (size_t)pp.ni * pp.nj * pp.nk2));
+    // the fused j pass keeps a whole (j, k) slab in LDS: FP32 only (FP64 slabs of this size do not fit)
+    int dev = 0, lds = 0;
+    DYN_HIP(hipGetDevice(&dev));
+    DYN_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    pp.lds_max = (size_t)lds;
+    pp.fused_y = c->prec == DYN_FP32;
+    if (pp.fused_y) {
+        size_t need = 2 * (size_t)pp.nj * (pp.nk2 + 1) * sizeof(float) + 8 + 16 * (size_t)pp.nk2 + 8;
+        if (need > pp.lds_max) pp.fused_y = false;
+        else DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    }
     DYN_HIP(hipMalloc(&pp.ffac, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
     DYN_HIP(hipMalloc(&pp.coskx, sizeof(double) * pp.ni));
     DYN_HIP(hipMalloc(&pp.cosly, sizeof(double) * pp.nj));
@@ -488,8 +848,8 @@ int pressure_free(Ctx *c) {
     return DYN_OK;
 }
 
-int launch_solve_pressure(Ctx *c, double step) {
-    return c->prec == DYN_FP32 ? solve_t<float>(c, step) : solve_t<double>(c, step);
+int launch_solve_pressure(Ctx *c, double step, bool halos) {
+    return c->prec == DYN_FP32 ? solve_t<float>(c, step, halos) : solve_t<double>(c, step, halos);
 }
 
 }  // namespace dyn
