@@ -31,9 +31,10 @@ PASSES = {
     "advect_uvw": {0: 13, 1: 16}, "advect_theta": 10, "advect_moisture": 16, "fall_speed": 2, "zero_moisture": 7,
     "pgf": 7, "p1_div_fft_x": 4, "p2_fft_y": 2, "p3_thomas": 2, "p4_ifft_y": 2, "p5_ifft_x_out": 2, "kessler": 12,
     "rayleigh": 8, "advance_copy": 2,
-    # fused FP32 stage: current U V W TH QV QC QR, basic UB VB WB THB QB, M buffers (7) read; 7 forecasts
-    # and the fall speed written
-    "fused_advect": {0: 8 + 4 + 4 + 4, 1: 7 + 5 + 7 + 8},
+    # fused stage, momentum kernel: U V W UB VB WB cur/basic, UM VM WM, theta' (+ qv qc qr) read; UP VP WP written
+    "adv_momentum": {0: 6 + 3 + 1 + 3, 1: 6 + 3 + 4 + 3},
+    # scalar kernel: U V W UB VB WB, theta' thetabar (+ qv qbar qc qr), M buffers read; forecasts (+ VT) written
+    "adv_scalars": {0: 6 + 2 + 1 + 1, 1: 6 + 6 + 4 + 5},
     "p1_div_rfft_x": 4, "p2_ypass": 2, "p5_irfft_x_out": 2,
     "post_uvw": 7, "post_scalars": 1, "zero_moisture_shell": 1,
 }
@@ -135,7 +136,7 @@ def main():
         log(f"bench: {name} {ms:.3f} ms/step, {res[name]['value']:.4g} gridpoint-steps/s")
     main_form = forms[0][0]
     g = handles[main_form]
-    rows = kernel_profile(g, args.profile_steps, fused=(main_form == "fp32"))
+    rows = kernel_profile(g, args.profile_steps, fused=True)
     for r in sorted(rows, key=lambda r: -r[2]):
         log(f"bench: kernel {r[0]:16s} launches {r[1]:5d} total {r[2]:9.3f} ms avg {r[2] / r[1] * 1e3:8.1f} us")
     roof = roofline(rows, 4 if main_form == "fp32" else 8, case.microphysics)

@@ -102,11 +102,11 @@ int dyn_micro_advance_outer(dyn_ctx *ctx);
 /* ---- whole steps ------------------------------------------------------------------------- */
 /* s_integrate_rk3 (driver_serial.cpp:178-286) as the sequence of per-operator calls above */
 int dyn_step_reference(dyn_ctx *ctx, int nsteps);
-/* the step replayed from a captured hipGraph: for DYN_FP32 contexts the stage advection runs as one
- * fused kernel (same values as the per-operator kernels, bit for bit); DYN_FP64 contexts replay the
- * per-operator sequence */
+/* the step replayed from a captured hipGraph: when NZ <= 64 the stage advection runs as the fused
+ * momentum and scalar kernels with fused boundary epilogues (same values as the per-operator kernels,
+ * bit for bit, in either precision); otherwise the per-operator sequence is replayed */
 int dyn_step(dyn_ctx *ctx, int nsteps);
-/* the fused-stage step launched eagerly (either precision; needs NZ <= 64) */
+/* the fused-stage step launched eagerly (either precision; needs NZ <= 64 and fields below 4 GiB) */
 int dyn_step_fused(dyn_ctx *ctx, int nsteps);
 
 /* ---- measurement ------------------------------------------------------------------------- */
@@ -119,6 +119,20 @@ int dyn_timing(dyn_ctx *ctx, int on);
 int dyn_timing_report(dyn_ctx *ctx, char *buf, size_t len);
 /* the HIP stream the context launches on (a hipStream_t) */
 void *dyn_stream(dyn_ctx *ctx);
+
+/* ---- launch-shape parameters ------------------------------------------------------------- */
+/* Knobs of the fused kernels (no counterpart in the reference: they set how the work maps onto the
+ * GPU, never what is computed).  0 restores the built-in default; a value the kernels do not support
+ * returns DYN_E_ARG.  Setting a knob drops the captured step graph. */
+enum {
+    DYN_TUNE_MOMENTUM_TJ = 0, /* j-rows per block of the momentum stage kernel: 2, 4 or 8 */
+    DYN_TUNE_MOMENTUM_TI = 1, /* columns one momentum block marches along i: 2 .. 512 */
+    DYN_TUNE_SCALAR_TJ = 2,   /* j-rows per block of the scalar (theta, moisture) stage kernel: 2, 4 or 8 */
+    DYN_TUNE_SCALAR_TI = 3,   /* columns one scalar block marches along i: 2 .. 512 */
+    DYN_TUNE_COUNT = 4
+};
+int dyn_tune_set(dyn_ctx *ctx, int param, int value);
+int dyn_tune_get(dyn_ctx *ctx, int param);
 
 #ifdef __cplusplus
 }

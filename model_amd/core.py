@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "libdyncore.so")
 REFAPI_PATH = os.path.join(_HERE, "_build", "libdyncore_refapi.so")
 
 DYN_FP64, DYN_FP32 = 0, 1
+DYN_TUNE_MOMENTUM_TJ, DYN_TUNE_MOMENTUM_TI, DYN_TUNE_SCALAR_TJ, DYN_TUNE_SCALAR_TI = 0, 1, 2, 3
 ERRORS = {-1: "DYN_E_ARG", -2: "DYN_E_UNSUPPORTED", -3: "DYN_E_HIP", -4: "DYN_E_NUMERIC", -5: "DYN_E_STATE"}
 
 # the exported C symbols, as declared in include/dyncore.h
@@ -26,7 +27,7 @@ EXPORTS = [
     "dyn_solve_pressure", "dyn_pgf", "dyn_apply_bc", "dyn_apply_bc_micro", "dyn_kessler", "dyn_rayleigh",
     "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner", "dyn_micro_advance_outer",
     "dyn_step_reference", "dyn_step", "dyn_step_fused", "dyn_algorithmic_bytes_per_point", "dyn_timing", "dyn_timing_report",
-    "dyn_stream",
+    "dyn_stream", "dyn_tune_set", "dyn_tune_get",
 ]
 
 
@@ -79,6 +80,8 @@ def load():
     L.dyn_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_timing_report.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     L.dyn_stream.argtypes = [ctypes.c_void_p]
+    L.dyn_tune_set.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.dyn_tune_get.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_stream.restype = ctypes.c_void_p
     _lib = L
     return L
@@ -161,6 +164,11 @@ class DynCore:
 
     def bytes_per_point(self):
         return self.L.dyn_algorithmic_bytes_per_point(self.h)
+
+    def tune_get(self, param: int) -> int:
+        v = self.L.dyn_tune_get(self.h, param)
+        _check(min(v, 0))
+        return v
 
     def __getattr__(self, name):
         if name.startswith("_") or name in ("L", "h", "case", "ids"):

@@ -47,6 +47,15 @@ int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip);
 
 namespace {
 
+// the captured step graph bakes in launch shapes: drop it when a knob changes
+void drop_graph(Ctx *c) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
 __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long long n) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long s = (long long)gridDim.x * blockDim.x;
@@ -608,7 +617,7 @@ static int fused_step(dyn_ctx *ctx) {
 // the sequence dyn_step records: fused stages for the FP32 form, per-operator kernels for FP64
 static int graph_step(dyn_ctx *ctx) {
     Ctx *c = as_ctx(ctx);
-    if (c->prec == DYN_FP32 && fused_supported(c)) return fused_step(ctx);
+    if (fused_supported(c)) return fused_step(ctx);
     return reference_step(ctx);
 }
 
@@ -617,7 +626,7 @@ int dyn_step_fused(dyn_ctx *ctx, int nsteps) {
     CHECK_CTX(c);
     CHECK_COLS(c);
     if (!fused_supported(c)) {
-        set_error("dyn_step_fused: the fused stage kernel needs NZ <= 64");
+        set_error("dyn_step_fused: the fused stage kernels need NZ <= 64 and fields below 4 GiB");
         return DYN_E_UNSUPPORTED;
     }
     if (c->g.rayleigh_level < c->g.nz) {
@@ -758,6 +767,28 @@ double dyn_algorithmic_bytes_per_point(const dyn_ctx *ctx) {
 void *dyn_stream(dyn_ctx *ctx) {
     Ctx *c = as_ctx(ctx);
     return c ? (void *)c->stream : nullptr;
+}
+
+int dyn_tune_set(dyn_ctx *ctx, int param, int value) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    if (param < 0 || param >= DYN_TUNE_COUNT) {
+        set_error("dyn_tune_set: unknown parameter " + std::to_string(param));
+        return DYN_E_ARG;
+    }
+    if (value != 0 && !fused_shape_ok(c, param, value)) {
+        set_error("dyn_tune_set: value " + std::to_string(value) + " not supported for parameter " + std::to_string(param));
+        return DYN_E_ARG;
+    }
+    c->tune[param] = value;
+    drop_graph(c);
+    return DYN_OK;
+}
+
+int dyn_tune_get(dyn_ctx *ctx, int param) {
+    Ctx *c = as_ctx(ctx);
+    if (!c || param < 0 || param >= DYN_TUNE_COUNT) return DYN_E_ARG;
+    return fused_shape(c, param);
 }
 
 }  // extern "C"

@@ -37,221 +37,499 @@ __device__ __forceinline__ T i3(T pm1, T p0, T p1, T p2, T sign) {
 template <typename T>
 __device__ __forceinline__ T i2(T p0, T p1) { return T(0.5) * (p1 + p0); }
 
-// queue fields (register, own j,k, columns i-3..i+3)
-enum { QU = 0, QUB, QV, QVB, QTH, QTHB, QQV, QQB, QQC, QQR, NQ };
-// LDS plane fields
-enum { LU = 0, LUB, LV, LVB, LW, LWB, LTH, LTHB, LQV, LQB, LQC, LQR, NL };
-// face quantities
-enum { FU = 0, FV, FTH, FQV, FQC, FQR, NF };
+// Element offsets are 32-bit (the host checks that a field is below 4 GiB): the base pointer stays in
+// SGPRs and one VGPR byte offset serves every field at the same point.
+template <typename T>
+__device__ __forceinline__ T ld(const void *base, unsigned idx) {
+    return *(const T *)((const char *)base + (size_t)(idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st(void *base, unsigned idx, T v) {
+    *(T *)((char *)base + (size_t)(idx * (unsigned)sizeof(T))) = v;
+}
 
-struct FusedArgs {
+// Workgroups are dispatched round-robin over the 8 XCDs.  Tile t = jblock + nbj * ichunk; hardware block
+// b is given a tile so that each XCD owns a contiguous run of tiles: neighbouring j-blocks of one i-chunk
+// (which share halo rows) then sit behind the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n / 8, rem = n % 8, x = b % 8;
+    return x * q + min(x, rem) + b / 8;
+}
+
+struct StageArgs {
     const void *cur[7];   // U V W TH QV QC QR (current)
     const void *mprev[7]; // M buffers
     void *out[7];         // P buffers
     const void *bar[5];   // UB VB WB THB QB
     void *vt;
     int il, ih, jl, jh;   // advected box
-    int ti;               // columns per block
+    int ti;               // columns one block marches
+    int nbj;              // j-blocks
 };
 
-template <typename T, int TJ, bool MOIST>
-struct Smem {
+// ---- momentum: advect_uvw_velocity (u, v fluxes and Coriolis; w = buoyancy, advection.cpp:304-339) -------
+// LDS: U UB (planes i, i+1) and V VB (planes i-1, i) on rows j0-3 .. j0+TJ+2, W WB (planes i-1, i) on rows
+// j0-1 .. j0+TJ-1, each in a ring of three slots (plane p in slot p % 3); north faces of rows j0-1 .. j0+TJ-1.
+// Register queues hold U UB V VB of the own row at columns i-2 .. i+3.
+template <typename T, int TJ>
+struct MomLds {
     static constexpr int R = TJ + 6;
-    // ring of 3 planes for U,UB,V,VB,W,WB and one plane for the scalars
-    static constexpr int NWIN = 6;
-    static constexpr int NSCAL = MOIST ? 6 : 2;
-    static constexpr size_t plane = (size_t)R * KP;
-    static constexpr size_t win_elems = 3 * NWIN * plane;
-    static constexpr size_t scal_elems = NSCAL * plane;
-    static constexpr size_t north_elems = NF * (TJ + 1) * KP;
-    static constexpr size_t top_elems = NF * TJ * KP;
-    static constexpr size_t vt_elems = TJ * KP;
-    static constexpr size_t total = win_elems + scal_elems + north_elems + top_elems + vt_elems;
-    static constexpr size_t bytes = total * sizeof(T);
+    static constexpr int RW = TJ + 1;
+    static constexpr size_t uv = 3 * 4 * (size_t)R * KP;    // [slot][U UB V VB][row][k]
+    static constexpr size_t w = 3 * 2 * (size_t)RW * KP;    // [slot][W WB][row][k]
+    static constexpr size_t nf = 2 * (size_t)(TJ + 1) * KP; // [u v][row + 1][k]
+    static constexpr size_t bytes = (uv + w + nf) * sizeof(T);
 };
 
 template <typename T, int TJ, bool MOIST>
-__global__ void __launch_bounds__(KP *TJ) k_stage_advect(Geo g, Cols c, FusedArgs a, T step) {
-    using S = Smem<T, TJ, MOIST>;
+__global__ void __launch_bounds__(KP *TJ) k_adv_momentum(Geo g, Cols c, StageArgs a, T step) {
+    using L = MomLds<T, TJ>;
+    constexpr int R = L::R, RW = L::RW;
+    constexpr int HU = (R + TJ - 1) / TJ, HW = (RW + TJ - 1) / TJ;  // plane rows per wavefront
     extern __shared__ unsigned char smem_raw[];
-    T *sm = (T *)smem_raw;
-    T *win = sm;                       // [slot][field][row][k]
-    T *scal = win + S::win_elems;      // [field][row][k]
-    T *nface = scal + S::scal_elems;   // [face][row+1][k]
-    T *tface = nface + S::north_elems; // [face][row][k]
-    T *vtp = tface + S::top_elems;     // [row][k]
+    T *uvp = (T *)smem_raw;
+    T *wpl = uvp + L::uv;
+    T *nface = wpl + L::w;
 
-    const int k = threadIdx.x;
-    const int r = threadIdx.y;
-    const int j0 = a.jl + blockIdx.x * TJ;
-    const int j = j0 + r;
-    const int i0 = a.il + blockIdx.y * a.ti;
+    const int k = threadIdx.x, r = threadIdx.y;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int j0 = a.jl + (t % a.nbj) * TJ;
+    const int i0 = a.il + (t / a.nbj) * a.ti;
     const int i1 = min(i0 + a.ti, a.ih);
-    if (i0 >= i1) return;
-    const int NZ = g.NZ;
-    const bool kin = k < NZ;
-    const bool row_ok = j < a.jh;
-    const bool active = row_ok && k >= 1 && k <= NZ - 2;
-    const int kc = kin ? k : NZ - 1;  // clamped level for loads
-    const int jc = min(j, g.NY - 1);
+    const int NZ = g.NZ, NY = g.NY;
+    const unsigned NYNZ = (unsigned)g.NYNZ;
+    const int j = j0 + r;
+    const bool active = j < a.jh && k >= 1 && k <= NZ - 2;
+    const int kc = min(k, NZ - 1), jc = min(j, NY - 1);
+    const int km1 = max(kc - 1, 0), kp1 = min(kc + 1, NZ - 1), kp2 = min(kc + 2, NZ - 1);
+    const unsigned rowoff = (unsigned)(jc * NZ + kc);
+    const int RO = r + 3, WO = r + 1;  // own row in the U/V and W planes
 
-    const T *U = (const T *)a.cur[0], *V = (const T *)a.cur[1], *W = (const T *)a.cur[2], *TH = (const T *)a.cur[3];
-    const T *QVf = (const T *)a.cur[4], *QCf = (const T *)a.cur[5], *QRf = (const T *)a.cur[6];
-    const T *UB = (const T *)a.bar[0], *VB = (const T *)a.bar[1], *WB = (const T *)a.bar[2], *THB = (const T *)a.bar[3];
-    const T *QB = (const T *)a.bar[4];
-    const T *winsrc[6] = {U, UB, V, VB, W, WB};
-    const T *scalsrc[6] = {TH, THB, QVf, QB, QCf, QRf};
-    const T *qsrc[NQ] = {U, UB, V, VB, TH, THB, QVf, QB, QCf, QRf};
-    constexpr int nq = MOIST ? NQ : 6;
+    auto UV = [&](int slot, int f, int row) -> T * { return uvp + ((slot * 4 + f) * R + row) * KP; };
+    auto WW = [&](int slot, int f, int row) -> T * { return wpl + ((slot * 2 + f) * RW + row) * KP; };
+    const void *qsrc[4] = {a.cur[0], a.bar[0], a.cur[1], a.bar[1]};
 
-    auto W_AT = [&](int slot, int f, int row, int kk) -> T & { return win[((slot * 6 + f) * S::R + row) * KP + kk]; };
-    auto S_AT = [&](int f, int row, int kk) -> T & { return scal[(f * S::R + row) * KP + kk]; };
-    auto slot_of = [](int i) { return ((i % 3) + 3) % 3; };
-    auto gidx = [&](int i, int jj, int kk) -> long long { return (long long)i * g.NYNZ + (long long)jj * NZ + kk; };
-
-    // cooperative load of one plane of a set of fields (rows j0-3 .. j0+TJ+2)
-    auto load_win_plane = [&](int i) {
-        int slot = slot_of(i);
-        for (int rr = r; rr < S::R; rr += TJ) {
-            int jj = min(max(j0 - 3 + rr, 0), g.NY - 1);
-            long long p = gidx(i, jj, kc);
+    // plane rows: U UB of plane iu, V VB W WB of plane iv
+    T su[HU][4], sw[HW][2];
+    auto fetch_planes = [&](int iu, int iv) {
 #pragma unroll
-            for (int f = 0; f < 6; f++) W_AT(slot, f, rr, k) = winsrc[f][p];
+        for (int h = 0; h < HU; h++) {
+            const int rr = r + h * TJ;
+            if (rr < R) {
+                const int jj = min(max(j0 - 3 + rr, 0), NY - 1);
+                const unsigned pu = (unsigned)iu * NYNZ + (unsigned)(jj * NZ + kc), pv = (unsigned)iv * NYNZ + (unsigned)(jj * NZ + kc);
+                su[h][0] = ld<T>(a.cur[0], pu);
+                su[h][1] = ld<T>(a.bar[0], pu);
+                su[h][2] = ld<T>(a.cur[1], pv);
+                su[h][3] = ld<T>(a.bar[1], pv);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < HW; h++) {
+            const int rr = r + h * TJ;
+            if (rr < RW) {
+                const int jj = min(max(j0 - 1 + rr, 0), NY - 1);
+                const unsigned pv = (unsigned)iv * NYNZ + (unsigned)(jj * NZ + kc);
+                sw[h][0] = ld<T>(a.cur[2], pv);
+                sw[h][1] = ld<T>(a.bar[2], pv);
+            }
         }
     };
-    auto load_scal_plane = [&](int i) {
-        for (int rr = r; rr < S::R; rr += TJ) {
-            int jj = min(max(j0 - 3 + rr, 0), g.NY - 1);
-            long long p = gidx(i, jj, kc);
+    auto store_planes = [&](int iu, int iv) {
+        const int su_slot = iu % 3, sv_slot = iv % 3;
 #pragma unroll
-            for (int f = 0; f < S::NSCAL; f++) S_AT(f, rr, k) = scalsrc[f][p];
+        for (int h = 0; h < HU; h++) {
+            const int rr = r + h * TJ;
+            if (rr < R) {
+                UV(su_slot, 0, rr)[k] = su[h][0];
+                UV(su_slot, 1, rr)[k] = su[h][1];
+                UV(sv_slot, 2, rr)[k] = su[h][2];
+                UV(sv_slot, 3, rr)[k] = su[h][3];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < HW; h++) {
+            const int rr = r + h * TJ;
+            if (rr < RW) {
+                WW(sv_slot, 0, rr)[k] = sw[h][0];
+                WW(sv_slot, 1, rr)[k] = sw[h][1];
+            }
+        }
+    };
+    // own-column inputs of the buoyancy term (theta', qv, qc, qr at column i)
+    T cb[4] = {T(0), T(0), T(0), T(0)}, sb[4] = {T(0), T(0), T(0), T(0)};
+    auto fetch_col = [&](int i, T *b) {
+        const unsigned p = (unsigned)i * NYNZ + rowoff;
+        b[0] = ld<T>(a.cur[3], p);
+        if (MOIST) {
+            b[1] = ld<T>(a.cur[4], p);
+            b[2] = ld<T>(a.cur[5], p);
+            b[3] = ld<T>(a.cur[6], p);
         }
     };
 
-    // register queues: q[f][d] holds column i-3+d (d = 0..6) at iteration i
-    T q[NQ][7];
+    // prologue: queues at columns i0-3 .. i0+2, planes U(i0, i0+1), V/W(i0-1, i0)
+    T q[4][6], qs[4];
 #pragma unroll
-    for (int f = 0; f < nq; f++)
+    for (int f = 0; f < 4; f++)
 #pragma unroll
-        for (int d = 0; d < 7; d++) q[f][d] = qsrc[f][gidx(i0 - 3 + d, jc, kc)];
-
-    load_win_plane(i0 - 1);
-    load_win_plane(i0);
-    load_win_plane(i0 + 1);
+        for (int d = 0; d < 6; d++) q[f][d] = ld<T>(qsrc[f], (unsigned)(i0 - 3 + d) * NYNZ + rowoff);
+    fetch_planes(i0, i0 - 1);
+    store_planes(i0, i0 - 1);
+    fetch_planes(i0 + 1, i0);
+    store_planes(i0 + 1, i0);
+    fetch_col(i0, cb);
 
     const T dtx = T(g.dtx), dty = T(g.dty), dt = T(g.dt);
     const T DTZ = T(g.dtz) * T(c.mu[kc]);
-    const T rw1 = T(c.rhow[min(kc + 1, NZ - 1)]), rw0 = T(c.rhow[kc]), odr = T(c.one_d_rhou[kc]);
+    const T rw1 = T(c.rhow[kp1]), rw0 = T(c.rhow[kc]), odr = T(c.one_d_rhou[kc]);
     const T fj = T(c.f[jc]);
-    const int R = r + 3;
+    const bool low = (k == 1 || k == NZ - 2);
 
-    // east-face fluxes of column i (queue positions: column i at d=3 after the shift below)
-    auto east_faces = [&](int base, T *e) {
-        // base: queue index of column "i" (the low side of the face i+1/2)
-#define QQ(f, o) q[f][base + (o)]
-        {  // u, u_east
-            T ua = T(0.5) * (QQ(QU, 1) + QQ(QU, 0));
-            T uba = T(0.5) * (QQ(QUB, 1) + QQ(QUB, 0));
-            T ue = i5(QQ(QU, -2), QQ(QU, -1), QQ(QU, 0), QQ(QU, 1), QQ(QU, 2), QQ(QU, 3), sgn(uba + ua));
-            T ube = i5(QQ(QUB, -2), QQ(QUB, -1), QQ(QUB, 0), QQ(QUB, 1), QQ(QUB, 2), QQ(QUB, 3), sgn(ua));
-            e[FU] = ua * ube + (uba + ua) * ue;
-        }
-        // th (s_east)
+    // east faces of the queue's centre column (u_east, v_east; fluxes.cpp:1384-1592); Ujm1/UBjm1: row j-1 of
+    // the U planes at the column east of the centre
+    auto east_faces = [&](const T *Ujm1, const T *UBjm1, T *e) {
         {
-            T U1 = QQ(QU, 1), UB1 = QQ(QUB, 1);
-            T se = i5(QQ(QTH, -2), QQ(QTH, -1), QQ(QTH, 0), QQ(QTH, 1), QQ(QTH, 2), QQ(QTH, 3), sgn(UB1 + U1));
-            T be = i5(QQ(QTHB, -2), QQ(QTHB, -1), QQ(QTHB, 0), QQ(QTHB, 1), QQ(QTHB, 2), QQ(QTHB, 3), sgn(U1));
-            e[FTH] = U1 * be + (UB1 + U1) * se;
-            if (MOIST) {
-                T sv = i5(QQ(QQV, -2), QQ(QQV, -1), QQ(QQV, 0), QQ(QQV, 1), QQ(QQV, 2), QQ(QQV, 3), sgn(UB1 + U1));
-                T bv = i5(QQ(QQB, -2), QQ(QQB, -1), QQ(QQB, 0), QQ(QQB, 1), QQ(QQB, 2), QQ(QQB, 3), sgn(U1));
-                e[FQV] = U1 * bv + (UB1 + U1) * sv;
-                T ub = UB1 + U1;
-                e[FQC] = ub * i5(QQ(QQC, -2), QQ(QQC, -1), QQ(QQC, 0), QQ(QQC, 1), QQ(QQC, 2), QQ(QQC, 3), sgn(ub));
-                e[FQR] = ub * i5(QQ(QQR, -2), QQ(QQR, -1), QQ(QQR, 0), QQ(QQR, 1), QQ(QQR, 2), QQ(QQR, 3), sgn(ub));
-            }
+            T ua = T(0.5) * (q[0][3] + q[0][2]);
+            T uba = T(0.5) * (q[1][3] + q[1][2]);
+            T ue = i5(q[0][0], q[0][1], q[0][2], q[0][3], q[0][4], q[0][5], sgn(uba + ua));
+            T ube = i5(q[1][0], q[1][1], q[1][2], q[1][3], q[1][4], q[1][5], sgn(ua));
+            e[0] = ua * ube + (uba + ua) * ue;
         }
-#undef QQ
+        {
+            T ua = T(0.5) * (q[0][3] + Ujm1[k]);
+            T uba = T(0.5) * (q[1][3] + UBjm1[k]);
+            T ve = i5(q[2][0], q[2][1], q[2][2], q[2][3], q[2][4], q[2][5], sgn(uba + ua));
+            T vbe = i5(q[3][0], q[3][1], q[3][2], q[3][3], q[3][4], q[3][5], sgn(ua));
+            e[1] = ua * vbe + (uba + ua) * ve;
+        }
     };
-    // v east face needs U at column i+1 and row j-1 (window plane i+1)
-    auto v_east_face = [&](int base, int i) -> T {
-        int sp = slot_of(i + 1);
-        T ua = T(0.5) * (q[QU][base + 1] + W_AT(sp, 0, R - 1, k));
-        T uba = T(0.5) * (q[QUB][base + 1] + W_AT(sp, 1, R - 1, k));
-        T ve = i5(q[QV][base - 2], q[QV][base - 1], q[QV][base], q[QV][base + 1], q[QV][base + 2], q[QV][base + 3], sgn(uba + ua));
-        T vbe = i5(q[QVB][base - 2], q[QVB][base - 1], q[QVB][base], q[QVB][base + 1], q[QVB][base + 2], q[QVB][base + 3], sgn(ua));
-        return ua * vbe + (uba + ua) * ve;
-    };
-
-    T west[NF], east[NF];
-    // west faces of the first column: the east faces of column i0-1 (compute_west*)
     __syncthreads();
-    east_faces(2, west);
-    west[FV] = v_east_face(2, i0 - 1);
+    T west[2], east[2];
+    east_faces(UV(i0 % 3, 0, RO - 1), UV(i0 % 3, 1, RO - 1), west);  // column i0-1
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+#pragma unroll
+        for (int d = 0; d < 5; d++) q[f][d] = q[f][d + 1];
+        q[f][5] = ld<T>(qsrc[f], (unsigned)(i0 + 3) * NYNZ + rowoff);
+    }
 
     for (int i = i0; i < i1; i++) {
-        const int s0 = slot_of(i), sm1 = slot_of(i - 1);
-        load_scal_plane(i);
-        __syncthreads();
-        // ---- faces of column i -----------------------------------------------------------------
-        east_faces(3, east);
-        east[FV] = v_east_face(3, i);
-        // north faces for rows r (and r-1 for the first row): rows of plane i
-        auto north_faces = [&](int RR, T *nf) {
-            {  // u_north
-                T va = T(0.5) * (W_AT(s0, 2, RR + 1, k) + W_AT(sm1, 2, RR + 1, k));
-                T vba = T(0.5) * (W_AT(s0, 3, RR + 1, k) + W_AT(sm1, 3, RR + 1, k));
-                T un = i5(W_AT(s0, 0, RR - 2, k), W_AT(s0, 0, RR - 1, k), W_AT(s0, 0, RR, k), W_AT(s0, 0, RR + 1, k),
-                          W_AT(s0, 0, RR + 2, k), W_AT(s0, 0, RR + 3, k), sgn(vba + va));
-                T ubn = i5(W_AT(s0, 1, RR - 2, k), W_AT(s0, 1, RR - 1, k), W_AT(s0, 1, RR, k), W_AT(s0, 1, RR + 1, k),
-                           W_AT(s0, 1, RR + 2, k), W_AT(s0, 1, RR + 3, k), sgn(va));
-                nf[FU] = va * ubn + (vba + va) * un;
-            }
-            {  // v_north
-                T va = T(0.5) * (W_AT(s0, 2, RR + 1, k) + W_AT(s0, 2, RR, k));
-                T vba = T(0.5) * (W_AT(s0, 3, RR + 1, k) + W_AT(s0, 3, RR, k));
-                T vn = i5(W_AT(s0, 2, RR - 2, k), W_AT(s0, 2, RR - 1, k), W_AT(s0, 2, RR, k), W_AT(s0, 2, RR + 1, k),
-                          W_AT(s0, 2, RR + 2, k), W_AT(s0, 2, RR + 3, k), sgn(vba + va));
-                T vbn = i5(W_AT(s0, 3, RR - 2, k), W_AT(s0, 3, RR - 1, k), W_AT(s0, 3, RR, k), W_AT(s0, 3, RR + 1, k),
-                           W_AT(s0, 3, RR + 2, k), W_AT(s0, 3, RR + 3, k), sgn(va));
-                nf[FV] = va * vbn + (vba + va) * vn;
-            }
-            T V1 = W_AT(s0, 2, RR + 1, k), VB1 = W_AT(s0, 3, RR + 1, k);
-            {  // s_north (theta)
-                T sn = i5(S_AT(0, RR - 2, k), S_AT(0, RR - 1, k), S_AT(0, RR, k), S_AT(0, RR + 1, k), S_AT(0, RR + 2, k),
-                          S_AT(0, RR + 3, k), sgn(VB1 + V1));
-                T bn = i5(S_AT(1, RR - 2, k), S_AT(1, RR - 1, k), S_AT(1, RR, k), S_AT(1, RR + 1, k), S_AT(1, RR + 2, k),
-                          S_AT(1, RR + 3, k), sgn(V1));
-                nf[FTH] = V1 * bn + (VB1 + V1) * sn;
-            }
-            if (MOIST) {
-                T sn = i5(S_AT(2, RR - 2, k), S_AT(2, RR - 1, k), S_AT(2, RR, k), S_AT(2, RR + 1, k), S_AT(2, RR + 2, k),
-                          S_AT(2, RR + 3, k), sgn(VB1 + V1));
-                T bn = i5(S_AT(3, RR - 2, k), S_AT(3, RR - 1, k), S_AT(3, RR, k), S_AT(3, RR + 1, k), S_AT(3, RR + 2, k),
-                          S_AT(3, RR + 3, k), sgn(V1));
-                nf[FQV] = V1 * bn + (VB1 + V1) * sn;
-                T vb = VB1 + V1;
-                nf[FQC] = vb * i5(S_AT(4, RR - 2, k), S_AT(4, RR - 1, k), S_AT(4, RR, k), S_AT(4, RR + 1, k), S_AT(4, RR + 2, k),
-                                  S_AT(4, RR + 3, k), sgn(vb));
-                nf[FQR] = vb * i5(S_AT(5, RR - 2, k), S_AT(5, RR - 1, k), S_AT(5, RR, k), S_AT(5, RR + 1, k), S_AT(5, RR + 2, k),
-                                  S_AT(5, RR + 3, k), sgn(vb));
-            }
-        };
-        T nf[NF];
-        north_faces(R, nf);
+        const bool more = i + 1 < i1;
+        const int c0 = i % 3, cm1 = (i + 2) % 3, cp1 = (i + 1) % 3;
+        const unsigned p = (unsigned)i * NYNZ + rowoff;
+        // loads for the next column go out first; this column's M values next
+        if (more) {
 #pragma unroll
-        for (int f = 0; f < NF; f++) nface[(f * (TJ + 1) + r + 1) * KP + k] = nf[f];
-        if (r == 0) {
-            T sf[NF];
-            north_faces(R - 1, sf);
-#pragma unroll
-            for (int f = 0; f < NF; f++) nface[(f * (TJ + 1)) * KP + k] = sf[f];
+            for (int f = 0; f < 4; f++) qs[f] = ld<T>(qsrc[f], (unsigned)(i + 4) * NYNZ + rowoff);
+            fetch_planes(i + 2, i + 1);
+            fetch_col(i + 1, sb);
         }
-        // fall speed of this level (kessler.cpp:222-252); level 0 copies level 1, NZ-1 comes from the array
-        T vtk = T(0);
+        const T um = ld<T>(a.mprev[0], p), vm = ld<T>(a.mprev[1], p), wm = ld<T>(a.mprev[2], p);
+
+        east_faces(UV(cp1, 0, RO - 1), UV(cp1, 1, RO - 1), east);
+        // north faces (u_north, v_north) of row RR of the U/V planes
+        auto north_faces = [&](int RR, T *nf) {
+            const T *U0 = UV(c0, 0, 0), *UB0 = UV(c0, 1, 0), *V0 = UV(c0, 2, 0), *VB0 = UV(c0, 3, 0);
+            const T *Vm = UV(cm1, 2, 0), *VBm = UV(cm1, 3, 0);
+#define PL(ptr, row) ptr[(row) * KP + k]
+            {
+                T va = T(0.5) * (PL(V0, RR + 1) + PL(Vm, RR + 1));
+                T vba = T(0.5) * (PL(VB0, RR + 1) + PL(VBm, RR + 1));
+                T un = i5(PL(U0, RR - 2), PL(U0, RR - 1), PL(U0, RR), PL(U0, RR + 1), PL(U0, RR + 2), PL(U0, RR + 3), sgn(vba + va));
+                T ubn = i5(PL(UB0, RR - 2), PL(UB0, RR - 1), PL(UB0, RR), PL(UB0, RR + 1), PL(UB0, RR + 2), PL(UB0, RR + 3), sgn(va));
+                nf[0] = va * ubn + (vba + va) * un;
+            }
+            {
+                T va = T(0.5) * (PL(V0, RR + 1) + PL(V0, RR));
+                T vba = T(0.5) * (PL(VB0, RR + 1) + PL(VB0, RR));
+                T vn = i5(PL(V0, RR - 2), PL(V0, RR - 1), PL(V0, RR), PL(V0, RR + 1), PL(V0, RR + 2), PL(V0, RR + 3), sgn(vba + va));
+                T vbn = i5(PL(VB0, RR - 2), PL(VB0, RR - 1), PL(VB0, RR), PL(VB0, RR + 1), PL(VB0, RR + 2), PL(VB0, RR + 3), sgn(va));
+                nf[1] = va * vbn + (vba + va) * vn;
+            }
+#undef PL
+        };
+        T nf[2];
+        north_faces(RO, nf);
+        nface[(0 * (TJ + 1) + r + 1) * KP + k] = nf[0];
+        nface[(1 * (TJ + 1) + r + 1) * KP + k] = nf[1];
+        if (r == 0) {
+            T sf[2];
+            north_faces(RO - 1, sf);
+            nface[(0 * (TJ + 1)) * KP + k] = sf[0];
+            nface[(1 * (TJ + 1)) * KP + k] = sf[1];
+        }
+        // top faces (u_top, v_top) of this level; level 0 has none
+        T tf[2];
+        {
+            const T *U0 = UV(c0, 0, RO), *UB0 = UV(c0, 1, RO);
+            const T *W0 = WW(c0, 0, WO), *WB0 = WW(c0, 1, WO), *Wm = WW(cm1, 0, WO), *WBm = WW(cm1, 1, WO);
+            T wa = T(0.5) * (W0[kp1] + Wm[kp1]);
+            T wba = T(0.5) * (WB0[kp1] + WBm[kp1]);
+            T ut, ubt;
+            if (low) {
+                ut = i2(U0[kc], U0[kp1]);
+                ubt = i2(UB0[kc], UB0[kp1]);
+            } else {
+                ut = i3(U0[km1], U0[kc], U0[kp1], U0[kp2], sgn(wba + wa));
+                ubt = i3(UB0[km1], UB0[kc], UB0[kp1], UB0[kp2], sgn(wa));
+            }
+            tf[0] = wa * ubt + (wba + wa) * ut;
+        }
+        {
+            const T *V0 = UV(c0, 2, RO), *VB0 = UV(c0, 3, RO);
+            const T *W0 = WW(c0, 0, WO), *WB0 = WW(c0, 1, WO), *Wj = WW(c0, 0, WO - 1), *WBj = WW(c0, 1, WO - 1);
+            T wa = T(0.5) * (W0[kp1] + Wj[kp1]);
+            T wba = T(0.5) * (WB0[kp1] + WBj[kp1]);
+            T vt, vbt;
+            if (low) {
+                vt = i2(V0[kc], V0[kp1]);
+                vbt = i2(VB0[kc], VB0[kp1]);
+            } else {
+                vt = i3(V0[km1], V0[kc], V0[kp1], V0[kp2], sgn(wba + wa));
+                vbt = i3(VB0[km1], VB0[kc], VB0[kp1], VB0[kp2], sgn(wa));
+            }
+            tf[1] = wa * vbt + (wba + wa) * vt;
+        }
+        if (k == 0) tf[0] = tf[1] = T(0);
+        // bottom face of level k = top face of level k-1; buoyancy of level k-1 (advection.cpp:42-58)
+        const T bu = __shfl(tf[0], km1), bv = __shfl(tf[1], km1);
+        T bk = cb[0] / T(c.tbv[kc]);
+        if (MOIST) bk = cb[0] / T(c.tbv[kc]) + T(0.61) * cb[1] - cb[2] - cb[3];
+        const T bkm = __shfl(bk, km1);
+        __syncthreads();
+        if (active) {
+            const T su0 = nface[(0 * (TJ + 1) + r) * KP + k], sv0 = nface[(1 * (TJ + 1) + r) * KP + k];
+            {  // u (advection.cpp:73-99)
+                const T *V0 = UV(c0, 2, 0), *Vm = UV(cm1, 2, 0);
+                T other = T(0.25) * (V0[RO * KP + k] + V0[(RO + 1) * KP + k] + Vm[RO * KP + k] + Vm[(RO + 1) * KP + k]);
+                T tend = -dtx * (east[0] - west[0]) - dty * (nf[0] - su0) - DTZ * (rw1 * tf[0] - rw0 * bu) * odr + dt * fj * other;
+                st<T>(a.out[0], p, um + step * tend);
+            }
+            {  // v (advection.cpp:104-128)
+                const T *U0 = UV(c0, 0, 0), *Up = UV(cp1, 0, 0);
+                T other = T(0.25) * (U0[RO * KP + k] + Up[RO * KP + k] + U0[(RO - 1) * KP + k] + Up[(RO - 1) * KP + k]);
+                T tend = -dtx * (east[1] - west[1]) - dty * (nf[1] - sv0) - DTZ * (rw1 * tf[1] - rw0 * bv) * odr - dt * fj * other;
+                st<T>(a.out[1], p, vm + step * tend);
+            }
+            if (k >= 2) st<T>(a.out[2], p, wm + step * dt * T(kGrav) * T(0.5) * (bk + bkm));  // advection.cpp:329-337
+        }
+        west[0] = east[0];
+        west[1] = east[1];
+        if (more) {
+            store_planes(i + 2, i + 1);
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+#pragma unroll
+                for (int d = 0; d < 5; d++) q[f][d] = q[f][d + 1];
+                q[f][5] = qs[f];
+            }
+#pragma unroll
+            for (int f = 0; f < 4; f++) cb[f] = sb[f];
+        }
+        __syncthreads();
+    }
+}
+
+// ---- scalars: advect_theta (+ advect_microphysics_cell, the fall speed and the interior zero_moisture) ----
+// LDS: theta' thetabar (qv qbar qc qr) of plane i on rows j0-3 .. j0+TJ+2 in a ring of two slots, north
+// faces of rows j0-1 .. j0+TJ-1.  Register queues: the same fields of the own row at columns i-2 .. i+3.
+// Own-column inputs (u at i+1, v at j+1, w, the M values) are loaded per wavefront.
+template <typename T, int TJ, bool MOIST>
+struct ScalLds {
+    static constexpr int R = TJ + 6;
+    static constexpr int NS = MOIST ? 6 : 2;
+    static constexpr int NFS = MOIST ? 4 : 1;
+    static constexpr size_t s = 2 * NS * (size_t)R * KP;          // [slot][field][row][k]
+    static constexpr size_t nf = NFS * (size_t)(TJ + 1) * KP;     // [face][row + 1][k]
+    static constexpr size_t bytes = (s + nf) * sizeof(T);
+};
+
+template <typename T, int TJ, bool MOIST>
+__global__ void __launch_bounds__(KP *TJ) k_adv_scalars(Geo g, Cols c, StageArgs a, T step) {
+    using L = ScalLds<T, TJ, MOIST>;
+    constexpr int R = L::R, NS = L::NS, NFS = L::NFS;
+    constexpr int HS = (R + TJ - 1) / TJ;
+    extern __shared__ unsigned char smem_raw[];
+    T *spl = (T *)smem_raw;
+    T *nface = spl + L::s;
+
+    const int k = threadIdx.x, r = threadIdx.y;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int j0 = a.jl + (t % a.nbj) * TJ;
+    const int i0 = a.il + (t / a.nbj) * a.ti;
+    const int i1 = min(i0 + a.ti, a.ih);
+    const int NZ = g.NZ, NY = g.NY;
+    const unsigned NYNZ = (unsigned)g.NYNZ;
+    const int j = j0 + r;
+    const bool active = j < a.jh && k >= 1 && k <= NZ - 2;
+    const int kc = min(k, NZ - 1), jc = min(j, NY - 1), jn = min(j + 1, NY - 1);
+    const int km1 = max(kc - 1, 0), kp1 = min(kc + 1, NZ - 1), kp2 = min(kc + 2, NZ - 1);
+    const unsigned rowoff = (unsigned)(jc * NZ + kc), rowoffn = (unsigned)(jn * NZ + kc);
+    const unsigned topoff = (unsigned)(jc * NZ + NZ - 1);
+    const int RO = r + 3;
+
+    auto SP = [&](int slot, int f) -> T * { return spl + (slot * NS + f) * R * KP; };
+    const void *src[6] = {a.cur[3], a.bar[3], a.cur[4], a.bar[4], a.cur[5], a.cur[6]};
+
+    T ss[HS][NS];
+    auto fetch_plane = [&](int i) {
+#pragma unroll
+        for (int h = 0; h < HS; h++) {
+            const int rr = r + h * TJ;
+            if (rr < R) {
+                const int jj = min(max(j0 - 3 + rr, 0), NY - 1);
+                const unsigned pp = (unsigned)i * NYNZ + (unsigned)(jj * NZ + kc);
+#pragma unroll
+                for (int f = 0; f < NS; f++) ss[h][f] = ld<T>(src[f], pp);
+            }
+        }
+    };
+    auto store_plane = [&](int i) {
+        const int slot = i % 2;
+#pragma unroll
+        for (int h = 0; h < HS; h++) {
+            const int rr = r + h * TJ;
+            if (rr < R) {
+#pragma unroll
+                for (int f = 0; f < NS; f++) SP(slot, f)[rr * KP + k] = ss[h][f];
+            }
+        }
+    };
+    // own-column inputs of column i: u, ubar at i+1; v, vbar at j+1 and j; w, wbar; VT at the top level
+    struct Col {
+        T u1, ub1, vn, vbn, vs, vbs, w0, wb0, vtt;
+    };
+    auto fetch_col = [&](int i, Col &o) {
+        const unsigned p = (unsigned)i * NYNZ + rowoff, pe = (unsigned)(i + 1) * NYNZ + rowoff, pn = (unsigned)i * NYNZ + rowoffn;
+        o.u1 = ld<T>(a.cur[0], pe);
+        o.ub1 = ld<T>(a.bar[0], pe);
+        o.vn = ld<T>(a.cur[1], pn);
+        o.vbn = ld<T>(a.bar[1], pn);
+        if (r == 0) {
+            o.vs = ld<T>(a.cur[1], p);
+            o.vbs = ld<T>(a.bar[1], p);
+        }
+        o.w0 = ld<T>(a.cur[2], p);
+        o.wb0 = ld<T>(a.bar[2], p);
+        if (MOIST) o.vtt = ld<T>(a.vt, (unsigned)i * NYNZ + topoff);
+    };
+
+    T q[NS][6], qs[NS];
+#pragma unroll
+    for (int f = 0; f < NS; f++)
+#pragma unroll
+        for (int d = 0; d < 6; d++) q[f][d] = ld<T>(src[f], (unsigned)(i0 - 3 + d) * NYNZ + rowoff);
+
+    const T dtx = T(g.dtx), dty = T(g.dty), dt = T(g.dt);
+    const T DTZ = T(g.dtz) * T(c.mu[kc]);
+    const T rw1 = T(c.rhow[kp1]), rw0 = T(c.rhow[kc]), odr = T(c.one_d_rhou[kc]);
+    const bool low = (k == 1 || k == NZ - 2);
+
+    // east faces (s_east; fluxes.cpp:797-1032) of the queue's centre column with u, ubar at the column east of it
+    auto east_faces = [&](T U1, T UB1, T *e) {
+        T se = i5(q[0][0], q[0][1], q[0][2], q[0][3], q[0][4], q[0][5], sgn(UB1 + U1));
+        T be = i5(q[1][0], q[1][1], q[1][2], q[1][3], q[1][4], q[1][5], sgn(U1));
+        e[0] = U1 * be + (UB1 + U1) * se;
         if (MOIST) {
+            T sv = i5(q[2][0], q[2][1], q[2][2], q[2][3], q[2][4], q[2][5], sgn(UB1 + U1));
+            T bv = i5(q[3][0], q[3][1], q[3][2], q[3][3], q[3][4], q[3][5], sgn(U1));
+            e[1] = U1 * bv + (UB1 + U1) * sv;
+            T ub = UB1 + U1;
+            e[2] = ub * i5(q[4][0], q[4][1], q[4][2], q[4][3], q[4][4], q[4][5], sgn(ub));
+            e[3] = ub * i5(q[5][0], q[5][1], q[5][2], q[5][3], q[5][4], q[5][5], sgn(ub));
+        }
+    };
+    T west[NFS], east[NFS];
+    {  // west faces of column i0: the east faces of column i0-1 (compute_west_scalar, fluxes.cpp:479-501)
+        const unsigned pe = (unsigned)i0 * NYNZ + rowoff;
+        east_faces(ld<T>(a.cur[0], pe), ld<T>(a.bar[0], pe), west);
+    }
+#pragma unroll
+    for (int f = 0; f < NS; f++) {
+#pragma unroll
+        for (int d = 0; d < 5; d++) q[f][d] = q[f][d + 1];
+        q[f][5] = ld<T>(src[f], (unsigned)(i0 + 3) * NYNZ + rowoff);
+    }
+    Col cc{}, cs{};
+    fetch_col(i0, cc);
+    fetch_plane(i0);
+    store_plane(i0);
+    __syncthreads();
+
+    for (int i = i0; i < i1; i++) {
+        const bool more = i + 1 < i1;
+        const int c0 = i % 2;
+        const unsigned p = (unsigned)i * NYNZ + rowoff;
+        if (more) {
+#pragma unroll
+            for (int f = 0; f < NS; f++) qs[f] = ld<T>(src[f], (unsigned)(i + 4) * NYNZ + rowoff);
+            fetch_col(i + 1, cs);
+            fetch_plane(i + 1);
+        }
+        T mth = ld<T>(a.mprev[3], p), mqv = T(0), mqc = T(0), mqr = T(0), qbp = T(0);
+        if (MOIST) {
+            mqv = ld<T>(a.mprev[4], p);
+            mqc = ld<T>(a.mprev[5], p);
+            mqr = ld<T>(a.mprev[6], p);
+            qbp = ld<T>(a.bar[4], p);
+        }
+
+        east_faces(cc.u1, cc.ub1, east);
+        // north faces (s_north) of row RR with v, vbar at the face
+        auto north_faces = [&](int RR, T V1, T VB1, T *nf) {
+#define PL(f, row) SP(c0, f)[(row) * KP + k]
+            T sn = i5(PL(0, RR - 2), PL(0, RR - 1), PL(0, RR), PL(0, RR + 1), PL(0, RR + 2), PL(0, RR + 3), sgn(VB1 + V1));
+            T bn = i5(PL(1, RR - 2), PL(1, RR - 1), PL(1, RR), PL(1, RR + 1), PL(1, RR + 2), PL(1, RR + 3), sgn(V1));
+            nf[0] = V1 * bn + (VB1 + V1) * sn;
+            if (MOIST) {
+                T sv = i5(PL(2, RR - 2), PL(2, RR - 1), PL(2, RR), PL(2, RR + 1), PL(2, RR + 2), PL(2, RR + 3), sgn(VB1 + V1));
+                T bv = i5(PL(3, RR - 2), PL(3, RR - 1), PL(3, RR), PL(3, RR + 1), PL(3, RR + 2), PL(3, RR + 3), sgn(V1));
+                nf[1] = V1 * bv + (VB1 + V1) * sv;
+                T vb = VB1 + V1;
+                nf[2] = vb * i5(PL(4, RR - 2), PL(4, RR - 1), PL(4, RR), PL(4, RR + 1), PL(4, RR + 2), PL(4, RR + 3), sgn(vb));
+                nf[3] = vb * i5(PL(5, RR - 2), PL(5, RR - 1), PL(5, RR), PL(5, RR + 1), PL(5, RR + 2), PL(5, RR + 3), sgn(vb));
+            }
+#undef PL
+        };
+        T nf[NFS];
+        north_faces(RO, cc.vn, cc.vbn, nf);
+#pragma unroll
+        for (int f = 0; f < NFS; f++) nface[(f * (TJ + 1) + r + 1) * KP + k] = nf[f];
+        if (r == 0) {
+            T sf[NFS];
+            north_faces(RO - 1, cc.vs, cc.vbs, sf);
+#pragma unroll
+            for (int f = 0; f < NFS; f++) nface[(f * (TJ + 1)) * KP + k] = sf[f];
+        }
+        // top faces (s_top) of this level from the own column of the plane; w, wbar at k+1 by lane exchange
+        const T W1 = __shfl(cc.w0, kp1), WB1 = __shfl(cc.wb0, kp1);
+        auto s_top = [&](int fs, int fb) -> T {
+            const T *S = SP(c0, fs) + RO * KP, *B = SP(c0, fb) + RO * KP;
+            T st_, bt;
+            if (low) {
+                st_ = i2(S[kc], S[kp1]);
+                bt = i2(B[kc], B[kp1]);
+            } else {
+                st_ = i3(S[km1], S[kc], S[kp1], S[kp2], sgn(WB1 + W1));
+                bt = i3(B[km1], B[kc], B[kp1], B[kp2], sgn(W1));
+            }
+            return W1 * bt + (WB1 + W1) * st_;
+        };
+        T tf[NFS];
+        tf[0] = s_top(0, 1);
+        T vtk = T(0), qrt = T(0);
+        if (MOIST) {
+            tf[1] = s_top(2, 3);
+            {
+                const T *S = SP(c0, 4) + RO * KP;
+                T wb = WB1 + W1;
+                T tq = low ? i2(S[kc], S[kp1]) : i3(S[km1], S[kc], S[kp1], S[kp2], sgn(wb));
+                tf[2] = wb * tq;
+            }
+            // fall speed of this level (kessler.cpp:222-252); level 0 stores level 1's, NZ-1 keeps the array's
             if (k >= 1 && k <= NZ - 2) {
-                T qr = S_AT(5, R, k);
+                T qr = q[5][2];
                 if (qr > T(1.0e-12)) {
                     T rk = T(c.rhou[k]);
                     T qrr = fmax(qr * T(0.001) * rk, T(0));
@@ -259,168 +537,80 @@ __global__ void __launch_bounds__(KP *TJ) k_stage_advect(Geo g, Cols c, FusedArg
                     vtk = T(36.34) * (pow(qrr, T(0.1364))) * vtden;
                 }
             } else if (k == NZ - 1) {
-                vtk = ((const T *)a.vt)[gidx(i, jc, k)];
+                vtk = cc.vtt;
             }
-            vtp[r * KP + k] = vtk;
-        }
-        // top faces of this level (k+1/2); level 0 has none
-        T tf[NF];
-        {
-            const int kp1 = min(kc + 1, NZ - 1), km1 = max(kc - 1, 0), kp2 = min(kc + 2, NZ - 1);
-            const bool low = (k == 1 || k == NZ - 2);
-            {  // u_top
-                T wa = T(0.5) * (W_AT(s0, 4, R, kp1) + W_AT(sm1, 4, R, kp1));
-                T wba = T(0.5) * (W_AT(s0, 5, R, kp1) + W_AT(sm1, 5, R, kp1));
-                T ut, ubt;
-                if (low) {
-                    ut = i2(W_AT(s0, 0, R, kc), W_AT(s0, 0, R, kp1));
-                    ubt = i2(W_AT(s0, 1, R, kc), W_AT(s0, 1, R, kp1));
-                } else {
-                    ut = i3(W_AT(s0, 0, R, km1), W_AT(s0, 0, R, kc), W_AT(s0, 0, R, kp1), W_AT(s0, 0, R, kp2), sgn(wba + wa));
-                    ubt = i3(W_AT(s0, 1, R, km1), W_AT(s0, 1, R, kc), W_AT(s0, 1, R, kp1), W_AT(s0, 1, R, kp2), sgn(wa));
-                }
-                tf[FU] = wa * ubt + (wba + wa) * ut;
-            }
-            {  // v_top
-                T wa = T(0.5) * (W_AT(s0, 4, R, kp1) + W_AT(s0, 4, R - 1, kp1));
-                T wba = T(0.5) * (W_AT(s0, 5, R, kp1) + W_AT(s0, 5, R - 1, kp1));
-                T vt, vbt;
-                if (low) {
-                    vt = i2(W_AT(s0, 2, R, kc), W_AT(s0, 2, R, kp1));
-                    vbt = i2(W_AT(s0, 3, R, kc), W_AT(s0, 3, R, kp1));
-                } else {
-                    vt = i3(W_AT(s0, 2, R, km1), W_AT(s0, 2, R, kc), W_AT(s0, 2, R, kp1), W_AT(s0, 2, R, kp2), sgn(wba + wa));
-                    vbt = i3(W_AT(s0, 3, R, km1), W_AT(s0, 3, R, kc), W_AT(s0, 3, R, kp1), W_AT(s0, 3, R, kp2), sgn(wa));
-                }
-                tf[FV] = wa * vbt + (wba + wa) * vt;
-            }
-            T W1 = W_AT(s0, 4, R, kp1), WB1 = W_AT(s0, 5, R, kp1);
-            auto s_top = [&](int fs, int fb) -> T {
-                T st, bt;
-                if (low) {
-                    st = i2(S_AT(fs, R, kc), S_AT(fs, R, kp1));
-                    bt = i2(S_AT(fb, R, kc), S_AT(fb, R, kp1));
-                } else {
-                    st = i3(S_AT(fs, R, km1), S_AT(fs, R, kc), S_AT(fs, R, kp1), S_AT(fs, R, kp2), sgn(WB1 + W1));
-                    bt = i3(S_AT(fb, R, km1), S_AT(fb, R, kc), S_AT(fb, R, kp1), S_AT(fb, R, kp2), sgn(W1));
-                }
-                return W1 * bt + (WB1 + W1) * st;
-            };
-            tf[FTH] = s_top(0, 1);
-            if (MOIST) {
-                tf[FQV] = s_top(2, 3);
-                T wb = WB1 + W1;
-                T t = low ? i2(S_AT(4, R, kc), S_AT(4, R, kp1)) : i3(S_AT(4, R, km1), S_AT(4, R, kc), S_AT(4, R, kp1), S_AT(4, R, kp2), sgn(wb));
-                tf[FQC] = wb * t;
-            }
+            // rain top face with the fall speed (fluxes.cpp:734, 780; 966-1030)
+            const T v1l = __shfl(vtk, 1);
+            const T vte = k == 0 ? v1l : vtk;
+            const T vk1 = __shfl(vte, kp1);
+            const T *QRr = SP(c0, 5) + RO * KP;
             if (k == 0) {
-#pragma unroll
-                for (int f = 0; f < NF; f++) tf[f] = T(0);
+                T t0 = T(0.5) * (QRr[0] + QRr[1]);
+                qrt = -T(0.5) * (v1l + v1l) * t0;
+            } else {
+                T wb = WB1 + W1;
+                T tq;
+                if (kc == 1 || kc == NZ - 2) tq = i2(QRr[kc], QRr[kp1]);
+                else tq = i3(QRr[km1], QRr[kc], QRr[kp1], QRr[kp2], sgn(WB1 + W1 - T(0.5) * (vte + vk1)));
+                qrt = (wb - T(0.5) * (vk1 + vte)) * tq;
             }
         }
+        if (k == 0) {
 #pragma unroll
-        for (int f = 0; f < NF; f++) tface[(f * TJ + r) * KP + k] = tf[f];
+            for (int f = 0; f < NFS; f++) tf[f] = T(0);
+        }
+        T bot[NFS];
+#pragma unroll
+        for (int f = 0; f < NFS; f++) bot[f] = __shfl(tf[f], km1);
+        const T qrb = MOIST ? __shfl(qrt, km1) : T(0);
         __syncthreads();
-        // rain top faces need both levels' fall speeds
-        T qrt = T(0), qrb = T(0);
-        if (MOIST) {
-            const int kp1 = min(kc + 1, NZ - 1), km1 = max(kc - 1, 0), kp2 = min(kc + 2, NZ - 1);
-            auto vt_at = [&](int kk) -> T { return kk == 0 ? vtp[r * KP + 1] : vtp[r * KP + kk]; };
-            auto qr_top = [&](int kk) -> T {
-                // top face of level kk of this column (fluxes.cpp:734, 780; 966-1030)
-                if (kk == 0) {
-                    T t0 = T(0.5) * (S_AT(5, R, 0) + S_AT(5, R, 1));
-                    return -T(0.5) * (vt_at(1) + vt_at(0)) * t0;
-                }
-                int a1 = min(kk + 1, NZ - 1), am1 = max(kk - 1, 0), a2 = min(kk + 2, NZ - 1);
-                T vk = vt_at(kk), vk1 = vt_at(a1);
-                T wb = W_AT(s0, 5, R, a1) + W_AT(s0, 4, R, a1);
-                T t;
-                if (kk == 1 || kk == NZ - 2) t = i2(S_AT(5, R, kk), S_AT(5, R, a1));
-                else t = i3(S_AT(5, R, am1), S_AT(5, R, kk), S_AT(5, R, a1), S_AT(5, R, a2), sgn(W_AT(s0, 5, R, a1) + W_AT(s0, 4, R, a1) - T(0.5) * (vk + vk1)));
-                return (wb - T(0.5) * (vk1 + vk)) * t;
-            };
-            if (active) {
-                qrt = qr_top(kc);
-                qrb = qr_top(km1);
-            }
-            (void)kp1;
-            (void)kp2;
-        }
-        // ---- tendencies ---------------------------------------------------------------------
         if (active) {
-            const long long p = gidx(i, j, k);
-            auto SOUTH = [&](int f) { return nface[(f * (TJ + 1) + r) * KP + k]; };
-            auto BOTTOM = [&](int f) { return tface[(f * TJ + r) * KP + k - 1]; };
-            {  // u
-                T other = T(0.25) * (W_AT(s0, 2, R, k) + W_AT(s0, 2, R + 1, k) + W_AT(sm1, 2, R, k) + W_AT(sm1, 2, R + 1, k));
-                T tend = -dtx * (east[FU] - west[FU]) - dty * (nf[FU] - SOUTH(FU)) - DTZ * (rw1 * tf[FU] - rw0 * BOTTOM(FU)) * odr +
-                         dt * fj * other;
-                ((T *)a.out[0])[p] = ((const T *)a.mprev[0])[p] + step * tend;
-            }
-            {  // v
-                const int sp = slot_of(i + 1);
-                T other = T(0.25) * (W_AT(s0, 0, R, k) + W_AT(sp, 0, R, k) + W_AT(s0, 0, R - 1, k) + W_AT(sp, 0, R - 1, k));
-                T tend = -dtx * (east[FV] - west[FV]) - dty * (nf[FV] - SOUTH(FV)) - DTZ * (rw1 * tf[FV] - rw0 * BOTTOM(FV)) * odr -
-                         dt * fj * other;
-                ((T *)a.out[1])[p] = ((const T *)a.mprev[1])[p] + step * tend;
-            }
-            if (k >= 2) {  // w: buoyancy (advection.cpp:329-334)
-                T bk, bkm;
-                if (!MOIST) {
-                    bk = S_AT(0, R, k) / T(c.tbv[k]);
-                    bkm = S_AT(0, R, k - 1) / T(c.tbv[k - 1]);
-                } else {
-                    bk = S_AT(0, R, k) / T(c.tbv[k]) + T(0.61) * S_AT(2, R, k) - S_AT(4, R, k) - S_AT(5, R, k);
-                    bkm = S_AT(0, R, k - 1) / T(c.tbv[k - 1]) + T(0.61) * S_AT(2, R, k - 1) - S_AT(4, R, k - 1) - S_AT(5, R, k - 1);
-                }
-                ((T *)a.out[2])[p] = ((const T *)a.mprev[2])[p] + step * dt * T(kGrav) * T(0.5) * (bk + bkm);
-            }
+            T south[NFS];
+#pragma unroll
+            for (int f = 0; f < NFS; f++) south[f] = nface[(f * (TJ + 1) + r) * KP + k];
             // Z_ADVECT_BASE (advection.cpp:16-18)
             auto zbase = [&](const double *base) -> T {
-                return (T(0.50) * (T(c.rhow[k + 1]) * W_AT(s0, 4, R, k + 1) * (T(base[k + 1]) - T(base[k])) +
-                                   T(c.rhow[k]) * W_AT(s0, 4, R, k) * (T(base[k]) - T(base[k - 1]))) *
+                return (T(0.50) * (T(c.rhow[k + 1]) * W1 * (T(base[k + 1]) - T(base[k])) +
+                                   T(c.rhow[k]) * cc.w0 * (T(base[k]) - T(base[k - 1]))) *
                         T(c.one_d_rhou[k]));
             };
-            {  // theta
-                T tend = -dtx * (east[FTH] - west[FTH]) - dty * (nf[FTH] - SOUTH(FTH)) -
-                         DTZ * (rw1 * tf[FTH] - rw0 * BOTTOM(FTH)) * odr - DTZ * zbase(c.tb);
-                ((T *)a.out[3])[p] = ((const T *)a.mprev[3])[p] + step * tend;
+            {  // theta (advection.cpp:152-168)
+                T tend = -dtx * (east[0] - west[0]) - dty * (nf[0] - south[0]) - DTZ * (rw1 * tf[0] - rw0 * bot[0]) * odr -
+                         DTZ * zbase(c.tb);
+                st<T>(a.out[3], p, mth + step * tend);
             }
-            if (MOIST) {
-                T tq = -dtx * (east[FQV] - west[FQV]) - dty * (nf[FQV] - SOUTH(FQV)) - DTZ * (rw1 * tf[FQV] - rw0 * BOTTOM(FQV)) * odr -
+            if (MOIST) {  // advection.cpp:197-253
+                T tq = -dtx * (east[1] - west[1]) - dty * (nf[1] - south[1]) - DTZ * (rw1 * tf[1] - rw0 * bot[1]) * odr -
                        DTZ * zbase(c.qb);
-                T qv = ((const T *)a.mprev[4])[p] + step * tq;
-                T tc = -dtx * (east[FQC] - west[FQC]) - dty * (nf[FQC] - SOUTH(FQC)) - DTZ * (rw1 * tf[FQC] - rw0 * BOTTOM(FQC)) * odr;
-                T qc = ((const T *)a.mprev[5])[p] + step * tc;
-                T tr = -dtx * (east[FQR] - west[FQR]) - dty * (nf[FQR] - SOUTH(FQR)) - DTZ * (rw1 * qrt - rw0 * qrb) * odr;
-                T qrv = ((const T *)a.mprev[6])[p] + step * tr;
+                T qv = mqv + step * tq;
+                T tc = -dtx * (east[2] - west[2]) - dty * (nf[2] - south[2]) - DTZ * (rw1 * tf[2] - rw0 * bot[2]) * odr;
+                T qc = mqc + step * tc;
+                T tr = -dtx * (east[3] - west[3]) - dty * (nf[3] - south[3]) - DTZ * (rw1 * qrt - rw0 * qrb) * odr;
+                T qrv = mqr + step * tr;
                 // zero_moisture for the advected points (microphysics.cpp:677-698)
                 if (qc < T(0)) qc = T(0);
                 if (qrv < T(0)) qrv = T(0);
-                T qbp = QB[p];
                 if (qv + qbp + T(c.qb[k]) < T(0)) qv = -(qbp + T(c.qb[k]));
-                ((T *)a.out[4])[p] = qv;
-                ((T *)a.out[5])[p] = qc;
-                ((T *)a.out[6])[p] = qrv;
-                T *VTo = (T *)a.vt;
-                VTo[p] = vtk;
-                if (k == 1) VTo[p - 1] = vtk;
+                st<T>(a.out[4], p, qv);
+                st<T>(a.out[5], p, qc);
+                st<T>(a.out[6], p, qrv);
+                st<T>(a.vt, p, vtk);
+                if (k == 1) st<T>(a.vt, p - 1, vtk);
             }
         }
-        // ---- advance the march ----------------------------------------------------------------
 #pragma unroll
-        for (int f = 0; f < NF; f++) west[f] = east[f];
-        __syncthreads();  // all reads of plane i-1 and of the scalar plane are done
-        if (i + 1 < i1) {
+        for (int f = 0; f < NFS; f++) west[f] = east[f];
+        if (more) {
+            store_plane(i + 1);
 #pragma unroll
-            for (int f = 0; f < nq; f++) {
+            for (int f = 0; f < NS; f++) {
 #pragma unroll
-                for (int d = 0; d < 6; d++) q[f][d] = q[f][d + 1];
-                q[f][6] = qsrc[f][gidx(i + 4, jc, kc)];
+                for (int d = 0; d < 5; d++) q[f][d] = q[f][d + 1];
+                q[f][5] = qs[f];
             }
-            load_win_plane(i + 2);
+            cc = cs;
         }
+        __syncthreads();
     }
 }
 
@@ -443,10 +633,35 @@ __global__ void k_zero_moisture_shell(Geo g, Cols c, T *__restrict__ QVP, T *__r
     }
 }
 
-template <typename T, int TJ>
+template <typename T, int TJ, bool MOIST>
+int momentum_t(Ctx *c, const StageArgs &a0, T step, int ti) {
+    StageArgs a = a0;
+    a.ti = ti;
+    a.nbj = (a.jh - a.jl + TJ - 1) / TJ;
+    const int nbi = (a.ih - a.il + ti - 1) / ti;
+    const size_t sh = MomLds<T, TJ>::bytes;
+    Timed _t(c, "adv_momentum", (double)(a.ih - a.il) * (a.jh - a.jl) * (c->geo.NZ - 2));
+    hipLaunchKernelGGL((k_adv_momentum<T, TJ, MOIST>), dim3(a.nbj * nbi), dim3(KP, TJ), sh, c->stream, c->geo, c->cols, a, step);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
+template <typename T, int TJ, bool MOIST>
+int scalars_t(Ctx *c, const StageArgs &a0, T step, int ti) {
+    StageArgs a = a0;
+    a.ti = ti;
+    a.nbj = (a.jh - a.jl + TJ - 1) / TJ;
+    const int nbi = (a.ih - a.il + ti - 1) / ti;
+    const size_t sh = ScalLds<T, TJ, MOIST>::bytes;
+    Timed _t(c, "adv_scalars", (double)(a.ih - a.il) * (a.jh - a.jl) * (c->geo.NZ - 2));
+    hipLaunchKernelGGL((k_adv_scalars<T, TJ, MOIST>), dim3(a.nbj * nbi), dim3(KP, TJ), sh, c->stream, c->geo, c->cols, a, step);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
+template <typename T, bool MOIST>
 int stage_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
-    const bool moist = c->g.microphysics != 0;
-    FusedArgs a{};
+    StageArgs a{};
     for (int f = 0; f < 7; f++) {
         a.cur[f] = c->prog[f][DYN_ROLE_CUR];
         a.mprev[f] = c->prog[f][DYN_ROLE_M];
@@ -458,19 +673,16 @@ int stage_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
     a.ih = ih;
     a.jl = jl;
     a.jh = jh;
-    a.ti = 16;
-    dim3 blk(KP, TJ), grd((jh - jl + TJ - 1) / TJ, (ih - il + a.ti - 1) / a.ti);
-    if (moist) {
-        size_t sh = Smem<T, TJ, true>::bytes;
-        Timed _t(c, "fused_advect", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
-        hipLaunchKernelGGL((k_stage_advect<T, TJ, true>), grd, blk, sh, c->stream, c->geo, c->cols, a, T(step));
-    } else {
-        size_t sh = Smem<T, TJ, false>::bytes;
-        Timed _t(c, "fused_advect", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
-        hipLaunchKernelGGL((k_stage_advect<T, TJ, false>), grd, blk, sh, c->stream, c->geo, c->cols, a, T(step));
-    }
-    DYN_LAUNCH_CHECK();
-    return DYN_OK;
+    const int mtj = fused_shape(c, DYN_TUNE_MOMENTUM_TJ), mti = fused_shape(c, DYN_TUNE_MOMENTUM_TI);
+    const int stj = fused_shape(c, DYN_TUNE_SCALAR_TJ), sti = fused_shape(c, DYN_TUNE_SCALAR_TI);
+    int rc;
+    if (mtj == 2) rc = momentum_t<T, 2, MOIST>(c, a, T(step), mti);
+    else if (mtj == 4) rc = momentum_t<T, 4, MOIST>(c, a, T(step), mti);
+    else rc = momentum_t<T, 8, MOIST>(c, a, T(step), mti);
+    if (rc) return rc;
+    if (stj == 2) return scalars_t<T, 2, MOIST>(c, a, T(step), sti);
+    if (stj == 4) return scalars_t<T, 4, MOIST>(c, a, T(step), sti);
+    return scalars_t<T, 8, MOIST>(c, a, T(step), sti);
 }
 
 template <typename T>
@@ -638,27 +850,67 @@ int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip) {
 
 }  // namespace
 
-bool fused_supported(const Ctx *c) { return c->g.nz <= KP; }
+// one wavefront per column (NZ <= 64) and 32-bit byte offsets into every field
+bool fused_supported(const Ctx *c) { return c->g.nz <= KP && (double)c->npts * (double)c->elt < 4294967296.0; }
 
-// the stage kernel uses more than 64 KiB of LDS: raise the per-launch limit once, outside any capture
-int fused_prepare(Ctx *c) {
+// launch-shape knobs of the stage kernels: rows per block (instantiated widths) and march length
+bool fused_shape_ok(const Ctx *c, int param, int value) {
     (void)c;
-    DYN_HIP(hipFuncSetAttribute((const void *)k_stage_advect<float, 8, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)Smem<float, 8, true>::bytes));
-    DYN_HIP(hipFuncSetAttribute((const void *)k_stage_advect<float, 8, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)Smem<float, 8, false>::bytes));
-    DYN_HIP(hipFuncSetAttribute((const void *)k_stage_advect<double, 4, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)Smem<double, 4, true>::bytes));
-    DYN_HIP(hipFuncSetAttribute((const void *)k_stage_advect<double, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)Smem<double, 4, false>::bytes));
+    if (param == DYN_TUNE_MOMENTUM_TJ || param == DYN_TUNE_SCALAR_TJ) return value == 2 || value == 4 || value == 8;
+    if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
+    return false;
+}
+
+int fused_shape(const Ctx *c, int param) {
+    if (c->tune[param]) return c->tune[param];
+    const bool f32 = c->prec == DYN_FP32;
+    switch (param) {
+    case DYN_TUNE_MOMENTUM_TJ: return f32 ? 8 : 4;
+    case DYN_TUNE_SCALAR_TJ: return f32 ? 8 : 4;
+    case DYN_TUNE_MOMENTUM_TI:
+    case DYN_TUNE_SCALAR_TI: return 16;
+    default: return DYN_E_ARG;
+    }
+}
+
+template <typename T, int TJ>
+int set_lds_limits() {
+    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, TJ, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)MomLds<T, TJ>::bytes));
+    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, TJ, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)MomLds<T, TJ>::bytes));
+    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, TJ, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ScalLds<T, TJ, true>::bytes));
+    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, TJ, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ScalLds<T, TJ, false>::bytes));
     return DYN_OK;
+}
+
+// the stage kernels may use more than 64 KiB of LDS: raise the per-launch limit once, outside any capture
+int fused_prepare(Ctx *c) {
+    if (c->prec == DYN_FP32) {
+        int rc = set_lds_limits<float, 2>();
+        if (!rc) rc = set_lds_limits<float, 4>();
+        if (!rc) rc = set_lds_limits<float, 8>();
+        return rc;
+    }
+    int rc = set_lds_limits<double, 2>();
+    if (!rc) rc = set_lds_limits<double, 4>();
+    if (!rc) rc = set_lds_limits<double, 8>();
+    return rc;
 }
 
 // advect_uvw_velocity + advect_theta (+ advect_microphysics_cell and the interior zero_moisture)
 // over the box [il,ih) x [jl,jh)
 int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh) {
-    if (c->prec == DYN_FP32) return stage_t<float, 8>(c, step, il, ih, jl, jh);
-    return stage_t<double, 4>(c, step, il, ih, jl, jh);
+    // the i-queues reach columns il-3 .. ih+2 and the planes rows jl-3 .. jh+2
+    if (il < 3 || ih > c->g.nx - 3 || jl < 3 || jh > c->g.ny - 3 || il >= ih || jl >= jh) {
+        set_error("fused stage: box outside [3, NX-3) x [3, NY-3)");
+        return DYN_E_ARG;
+    }
+    const bool moist = c->g.microphysics != 0;
+    if (c->prec == DYN_FP32) return moist ? stage_t<float, true>(c, step, il, ih, jl, jh) : stage_t<float, false>(c, step, il, ih, jl, jh);
+    return moist ? stage_t<double, true>(c, step, il, ih, jl, jh) : stage_t<double, false>(c, step, il, ih, jl, jh);
 }
 
 // PGF (optional) and the boundary update of every prognostic after launch_fused_advect; stands for

@@ -98,6 +98,8 @@ struct Ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_parity = -1;
+    // launch-shape knobs (DYN_TUNE_*), 0 = default
+    int tune[DYN_TUNE_COUNT] = {0};
     // timing
     bool timing = false;
     bool capturing = false;
@@ -134,5 +136,8 @@ int pressure_init(Ctx *c);
 int pressure_free(Ctx *c);
 int launch_solve_pressure(Ctx *c, double step, bool halos = true);
 int launch_convert(Ctx *c, void *dst, const double *src, size_t n, bool to_device_type);
+// launch shapes of the stage kernels (DYN_TUNE_*): validity and the value in force
+bool fused_shape_ok(const Ctx *c, int param, int value);
+int fused_shape(const Ctx *c, int param);
 
 }  // namespace dyn

@@ -209,3 +209,24 @@ def test_fused_fp64_against_oracle_supercell():
     for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR):
         e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
         assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
+
+
+@pytest.mark.parametrize("precision,shape", [(DYN_FP32, (2, 8, 4, 32)), (DYN_FP32, (4, 32, 8, 5)), (DYN_FP32, (8, 5, 2, 16)),
+                                             (0, (2, 16, 8, 7)), (0, (8, 7, 4, 64))],
+                         ids=["fp32-a", "fp32-b", "fp32-c", "fp64-a", "fp64-b"])
+def test_fused_launch_shapes_bitwise(precision, shape):
+    """Rows per block and march length change how the stage maps onto the GPU, never its values."""
+    from model_amd import core
+    c = configs.supercell(nx=40, ny=36, nz=30)
+    a, b = DynCore(c, precision=precision), DynCore(c, precision=precision)
+    knobs = (core.DYN_TUNE_MOMENTUM_TJ, core.DYN_TUNE_MOMENTUM_TI, core.DYN_TUNE_SCALAR_TJ, core.DYN_TUNE_SCALAR_TI)
+    for knob, v in zip(knobs, shape):
+        b.tune_set(knob, v)
+        assert b.tune_get(knob) == v
+    a.step_fused(4)
+    b.step_fused(4)
+    a.sync()
+    b.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR, ids.VT):
+        role = CUR if f in ids.PROGNOSTIC else 0
+        assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), ids.FIELD_NAMES[f]
