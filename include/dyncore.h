@@ -134,6 +134,21 @@ enum {
 int dyn_tune_set(dyn_ctx *ctx, int param, int value);
 int dyn_tune_get(dyn_ctx *ctx, int param);
 
+/* ---- discrete adjoint of the dry step ------------------------------------------------------ */
+/* No counterpart in the reference (it has no adjoint).  dyn_adjoint_record runs nsteps forward steps
+ * from the current state (leaving the context at the final state) and keeps a snapshot of the u, v, w,
+ * theta buffers (M, current, forecast) every `every` steps.  dyn_adjoint_backward takes the gradient
+ * of a scalar J with respect to the final current u, v, w, theta (host arrays of NX*NY*NZ doubles, NULL
+ * meaning zero) and returns dJ/d(initial field) for each (NULL to skip), where an initial field is the
+ * value set into all three buffers of that prognostic.  It recomputes each segment from its snapshot
+ * and runs the transposed kernels backwards; upwind signs are those of the recorded trajectory.  The
+ * context is left at the final state again.  Dry contexts only (microphysics 0); the trajectory memory
+ * is 12 fields per snapshot plus 12 fields per step of one segment. */
+int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every);
+int dyn_adjoint_backward(dyn_ctx *ctx, const double *seed_u, const double *seed_v, const double *seed_w, const double *seed_th,
+                         double *grad_u, double *grad_v, double *grad_w, double *grad_th);
+int dyn_adjoint_release(dyn_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,7 @@ EXPORTS = [
     "dyn_solve_pressure", "dyn_pgf", "dyn_apply_bc", "dyn_apply_bc_micro", "dyn_kessler", "dyn_rayleigh",
     "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner", "dyn_micro_advance_outer",
     "dyn_step_reference", "dyn_step", "dyn_step_fused", "dyn_algorithmic_bytes_per_point", "dyn_timing", "dyn_timing_report",
-    "dyn_stream", "dyn_tune_set", "dyn_tune_get",
+    "dyn_stream", "dyn_tune_set", "dyn_tune_get", "dyn_adjoint_record", "dyn_adjoint_backward", "dyn_adjoint_release",
 ]
 
 
@@ -82,6 +82,9 @@ def load():
     L.dyn_stream.argtypes = [ctypes.c_void_p]
     L.dyn_tune_set.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.dyn_tune_get.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dyn_adjoint_record.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.dyn_adjoint_backward.argtypes = [ctypes.c_void_p] + [_dp] * 8
+    L.dyn_adjoint_release.argtypes = [ctypes.c_void_p]
     L.dyn_stream.restype = ctypes.c_void_p
     _lib = L
     return L

@@ -259,6 +259,7 @@ int dyn_destroy(dyn_ctx *ctx) {
     for (auto e : c->pool) (void)hipEventDestroy(e);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    adjoint_release(c);
     pressure_free(c);
     for (auto &f : c->prog)
         for (void *p : f)
@@ -538,7 +539,7 @@ int dyn_micro_advance_outer(dyn_ctx *ctx) {
 }
 
 // s_integrate_rk3, driver_serial.cpp:178-286 (no diffusion, turbulence, terrain or heating)
-static int reference_step(dyn_ctx *ctx) {
+static int reference_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullptr) {
     Ctx *c = as_ctx(ctx);
     const double steps[] = {1. / 3., 0.5, 1.0};
     const int NX = c->g.nx, NY = c->g.ny;
@@ -550,6 +551,7 @@ static int reference_step(dyn_ctx *ctx) {
     } while (0)
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
+        if (hook) RC(hook(c, s, arg));
         // integrate_non_hydro(step,0,NX,0,NY), solver.cpp:87-138
         RC(dyn_advect_uvw(ctx, step, 3, NX - 3, 3, NY - 3));
         RC(dyn_solve_pressure(ctx, step));
@@ -582,7 +584,7 @@ static int reference_step(dyn_ctx *ctx) {
 // advect_microphysics_cell and zero_moisture's interior in one pass).  advect_theta and the moisture
 // advection read only the current-stage fields, which the pressure solve and the gradient force never
 // write, so they move ahead of solve_pressure without changing any value.
-static int fused_step(dyn_ctx *ctx) {
+static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullptr) {
     Ctx *c = as_ctx(ctx);
     const double steps[] = {1. / 3., 0.5, 1.0};
     const int NX = c->g.nx, NY = c->g.ny;
@@ -594,6 +596,7 @@ static int fused_step(dyn_ctx *ctx) {
     } while (0)
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
+        if (hook) RC(hook(c, s, arg));
         RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3));
         // the halo columns periodic_uvw_boundaries writes are read only by the divergence (which reads
         // UP(3) for UP(NX-3)); the boundary update overwrites them before anything else reads them
@@ -792,3 +795,11 @@ int dyn_tune_get(dyn_ctx *ctx, int param) {
 }
 
 }  // extern "C"
+
+namespace dyn {
+// one step in the fastest eager form, with a callback at the start of every stage
+int step_with_hook(Ctx *c, StageHook hook, void *arg) {
+    dyn_ctx *ctx = reinterpret_cast<dyn_ctx *>(c);
+    return fused_supported(c) ? fused_step(ctx, hook, arg) : reference_step(ctx, hook, arg);
+}
+}  // namespace dyn

@@ -74,6 +74,8 @@ struct TimerRec {
     double units;  // gridpoints the launch processes
 };
 
+struct AdjState;  // trajectory and adjoint buffers of the hand-written adjoint (dyn_adjoint.hip)
+
 struct Ctx {
     dyn_grid g{};
     Geo geo{};
@@ -98,6 +100,7 @@ struct Ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_parity = -1;
+    AdjState *adj = nullptr;
     // launch-shape knobs (DYN_TUNE_*), 0 = default
     int tune[DYN_TUNE_COUNT] = {0};
     // timing
@@ -135,7 +138,13 @@ int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level);
 int pressure_init(Ctx *c);
 int pressure_free(Ctx *c);
 int launch_solve_pressure(Ctx *c, double step, bool halos = true);
+int launch_pressure_adjoint(Ctx *c, double step, void *pihat, void *uh, void *vh, void *wh, void *r1, void *r2);
 int launch_convert(Ctx *c, void *dst, const double *src, size_t n, bool to_device_type);
+void adjoint_release(Ctx *c);
+// a whole RK3 step with a callback at the start of each stage (stage 0, 1, 2); a non-zero return aborts
+typedef int (*StageHook)(Ctx *c, int stage, void *arg);
+int step_with_hook(Ctx *c, StageHook hook, void *arg);
+bool fused_supported(const Ctx *c);
 // launch shapes of the stage kernels (DYN_TUNE_*): validity and the value in force
 bool fused_shape_ok(const Ctx *c, int param, int value);
 int fused_shape(const Ctx *c, int param);

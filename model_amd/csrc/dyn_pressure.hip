@@ -413,10 +413,12 @@ constexpr int kMaxBY = 4;   // and in the fused y pass
 
 // Q1: divergence3d_periodic prologue + forward real DFT along i; block = (j, chunk of kc levels, kc even).
 // The periodic column UP(NX-3) is read as UP(3), the value periodic_uvw_boundaries puts there.
+// With src != nullptr the transform reads a ready real field src(i+3, j, k) instead of the divergence
+// (the transposed solve of the adjoint).
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T *__restrict__ UP, const T *__restrict__ VP,
                                                          const T *__restrict__ WP, T *__restrict__ spec, Radix rad, const double *tw,
-                                                         int nk2, int kc, double step) {
+                                                         int nk2, int kc, double step, const T *__restrict__ src) {
     extern __shared__ unsigned char smem[];
     const int n = rad.n, P = kc / 2, nkx = n / 2 + 1;
     T *re = (T *)smem, *im = re + n * P;
@@ -428,7 +430,9 @@ __global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T 
         int b = idx % kc, i = idx / kc;
         int kk = kk0 + b;
         T d = T(0);
-        if (kk < nk2) {
+        if (kk < nk2 && src) {
+            d = src[IX(i + 3, j, kk + 1)];
+        } else if (kk < nk2) {
             int k = kk + 1, ia = i + 3;
             int ie = ia + 1 == g.NX - 3 ? 3 : ia + 1;
             T rhou = T(c.rhou[k]);
@@ -572,9 +576,58 @@ __device__ void thomas_column(const ThomasArgs &ta, int kx, int ky, T *x, int xs
     }
 }
 
+// Transpose of thomas_column (the adjoint's vertical solve), in place.  With the pivots z_k =
+// 1/(b_k - a_k F_{k-1}) and the multipliers F_k of the table, the forward map is p = L'^{-1} x / S
+// (L' lower bidiagonal: 1/z_k on the diagonal, a_k below) followed by x_k = p_k - F_k x_{k+1}; for the
+// (0,0) column p_{N} is replaced by 0.  The transpose runs the two bidiagonal solves transposed and in
+// reverse order: q_1 = x_1, q_k = x_k - F_{k-1} q_{k-1} (q_N = 0 for (0,0)), then
+// b_N = z_N q_N, b_k = z_k (q_k - a_{k+1} b_{k+1}), result b / S.  The singular last pivot of (0,0)
+// is never formed.
+template <typename T>
+__device__ void thomas_column_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs, int is) {
+    const int nk = ta.nk, N = nk - 2;
+    const long long fs = ta.nj;
+    const double *F = ta.ffac + (long long)kx * N * ta.nj + ky;
+    const double size2d = (double)(ta.ni * ta.nj);
+    const double ck = ta.coskx[kx], cl = ta.cosly[ky];
+    const bool m0 = (kx == 0 && ky == 0);
+    double qr = double(x[0]), qi = double(x[is]);
+    for (int k = 2; k <= N; k++) {
+        const double f = F[(k - 2) * fs];
+        const int o = (k - 1) * xs;
+        qr = double(x[o]) - f * qr;
+        qi = double(x[o + is]) - f * qi;
+        x[o] = T(qr);
+        x[o + is] = T(qi);
+    }
+    auto piv = [&](int k) -> double {
+        if (k == 1) return 1.0 / bcoef(1, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz);
+        return 1.0 / (bcoef(k, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz) - ta.acoef[k] * F[(k - 2) * fs]);
+    };
+    double br = 0.0, bi = 0.0;
+    {
+        const int o = (N - 1) * xs;
+        if (!m0) {
+            const double z = piv(N);
+            br = double(x[o]) * z;
+            bi = double(x[o + is]) * z;
+        }
+        x[o] = T(br / size2d);
+        x[o + is] = T(bi / size2d);
+    }
+    for (int k = N - 1; k >= 1; k--) {
+        const int o = (k - 1) * xs;
+        const double z = piv(k);
+        br = (double(x[o]) - ta.acoef[k + 1] * br) * z;
+        bi = (double(x[o + is]) - ta.acoef[k + 1] * bi) * z;
+        x[o] = T(br / size2d);
+        x[o + is] = T(bi / size2d);
+    }
+}
+
 // Q2 (fused): forward DFT along j, FP64 tridiagonal solve, backward DFT along j for one kx, with the
 // whole (j, k) slab of that kx resident in LDS
-template <typename T>
+template <typename T, bool TR = false>
 __global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, Radix rady, const double *twy, ThomasArgs ta,
                                                int pad) {
     extern __shared__ unsigned char smem[];
@@ -593,7 +646,8 @@ __global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, R
     if (threadIdx.x < nj) {
         int ky = threadIdx.x;
         // re and im of one column sit in separate planes: x[kk] = re[ky*P2 + kk], x[kk + is] = im[...]
-        thomas_column<T>(ta, kx, ky, re + ky * P2, 1, (int)(im - re), mean);
+        if (TR) thomas_column_t<T>(ta, kx, ky, re + ky * P2, 1, (int)(im - re));
+        else thomas_column<T>(ta, kx, ky, re + ky * P2, 1, (int)(im - re), mean);
     }
     __syncthreads();
     lds_fft_ip<T, kMaxBY>(re, im, rady, nk2, P2, 1, twy, +1);
@@ -605,7 +659,7 @@ __global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, R
 }
 
 // Q2 (staged): tridiagonal solve of tg columns of one kx staged through LDS (coalesced in and out)
-template <typename T>
+template <typename T, bool TR = false>
 __global__ void __launch_bounds__(64) k_thomas_staged(T *__restrict__ spec, int nk2, ThomasArgs ta) {
     extern __shared__ unsigned char smem[];
     const int tg = blockDim.x, P2 = 2 * nk2 + 1;
@@ -619,7 +673,10 @@ __global__ void __launch_bounds__(64) k_thomas_staged(T *__restrict__ spec, int 
         x[col * P2 + rem] = spec[base + idx];
     }
     __syncthreads();
-    if ((int)threadIdx.x < ncol) thomas_column<T>(ta, kx, ky0 + threadIdx.x, x + threadIdx.x * P2, 2, 1, mean);
+    if ((int)threadIdx.x < ncol) {
+        if (TR) thomas_column_t<T>(ta, kx, ky0 + threadIdx.x, x + threadIdx.x * P2, 2, 1);
+        else thomas_column<T>(ta, kx, ky0 + threadIdx.x, x + threadIdx.x * P2, 2, 1, mean);
+    }
     __syncthreads();
     for (int idx = threadIdx.x; idx < ncol * nk2 * 2; idx += blockDim.x) {
         int col = idx / (2 * nk2), rem = idx % (2 * nk2);
@@ -696,7 +753,8 @@ int solve_t(Ctx *c, double step, bool halos) {
         {
             Timed _t(c, "p1_div_rfft_x", pts);
             hipLaunchKernelGGL(k_div_rfft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
-                               (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kc, step);
+                               (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kc, step,
+                               (const T *)nullptr);
         }
         DYN_LAUNCH_CHECK();
         const int pad = 1;
@@ -769,6 +827,140 @@ int solve_t(Ctx *c, double step, bool halos) {
     return DYN_OK;
 }
 
+
+// ---- adjoint of the solve (transposed projection for the hand-written adjoint) -----------------------
+// Transpose of P5's output stage: the value v(i, j, kk) was written to PI at (i+3, j, kk+1), at the ghost
+// level 0 (kk = 0) or NZ-1 (kk = nk2-1), and at the periodic halo copies; its adjoint is the sum of the
+// adjoints of every location it was written to.
+template <typename T>
+__global__ void k_adj_pi_fold(Geo g, const T *__restrict__ PH, T *__restrict__ R, int nk2) {
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, ia = blockIdx.z + 3;
+    if (kk >= nk2) return;
+    const int shift = g.NX - 6, k = kk + 1;
+    const int ks[3] = {k, kk == 0 ? 0 : -1, kk == nk2 - 1 ? g.NZ - 1 : -1};
+    T v = T(0);
+    for (int t = 0; t < 3; t++) {
+        if (ks[t] < 0) continue;
+        v += PH[IX(ia, j, ks[t])];
+        if (ia >= shift && ia < shift + 3) v += PH[IX(ia - shift, j, ks[t])];
+        if (ia >= 3 && ia < 6) v += PH[IX(ia + shift, j, ks[t])];
+    }
+    R[IX(ia, j, k)] = v;
+}
+
+// Transpose of P1's divergence prologue (divergence3d_periodic, pressure.cpp:1390-1424): the adjoint of the
+// right-hand side r(ia, j, k) (rows 0 and NY-1 fold onto the mirrored rows 1 and NY-2) is spread back onto
+// the forecast velocities, one gathering thread per velocity point.
+template <typename T>
+__global__ void k_adj_div(Geo g, Cols c, const T *__restrict__ R, T *__restrict__ UH, T *__restrict__ VH, T *__restrict__ WH,
+                          double step) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, x = blockIdx.z;
+    if (k >= g.NZ) return;
+    const T sdt = T(step * g.dt);
+    auto Rf = [&](int ia, int yy, int kk) -> T {
+        if (ia < 3 || ia >= g.NX - 3 || yy < 1 || yy > g.NY - 2 || kk < 1 || kk > g.NZ - 2) return T(0);
+        T v = R[IX(ia, yy, kk)];
+        if (yy == 1) v += R[IX(ia, 0, kk)];
+        if (yy == g.NY - 2) v += R[IX(ia, g.NY - 1, kk)];
+        return v;
+    };
+    T uh = T(0), vh = T(0), wh = T(0);
+    if (x >= 3 && x < g.NX - 3 && k >= 1 && k <= g.NZ - 2) {
+        const T a = T(c.rhou[k]) * T(g.one_d_dx) / sdt, b = T(c.rhou[k]) * T(g.one_d_dy) / sdt;
+        // UP(x) enters column ia as UP(ie) with ie = ia + 1 (ie = 3 for ia = NX-4) and as UP(ia)
+        const int iae = x == 3 ? g.NX - 4 : x - 1;
+        uh = a * (Rf(iae, y, k) - Rf(x, y, k));
+        vh = b * (Rf(x, y - 1, k) - Rf(x, y, k));
+    }
+    if (x >= 3 && x < g.NX - 3) {
+        const T rw = T(c.rhow[k]), odz = T(g.one_d_dz);
+        if (k >= 2) wh += rw * (odz * T(c.mu[k - 1])) * Rf(x, y, k - 1) / sdt;
+        if (k >= 1 && k <= g.NZ - 2) wh -= rw * (odz * T(c.mu[k])) * Rf(x, y, k) / sdt;
+    }
+    const long long p = IX(x, y, k);
+    UH[p] += uh;
+    VH[p] += vh;
+    WH[p] += wh;
+}
+
+template <typename T>
+__global__ void k_zero_field(T *__restrict__ A, long long n) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) A[p] = T(0);
+}
+
+template <typename T>
+int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T *r2) {
+    PressurePlan &pp = c->pp;
+    const Geo &g = c->geo;
+    if (pp.ni % 2 != 0) {
+        set_error("adjoint pressure solve: needs an even ni (half-spectrum transform)");
+        return DYN_E_UNSUPPORTED;
+    }
+    Radix rx{}, ry{};
+    factorize(pp.ni, rx);
+    factorize(pp.nj, ry);
+    const double pts = (double)pp.ni * pp.nj * pp.nk2;
+    {
+        Timed _t(c, "adj_pi_fold", pts);
+        hipLaunchKernelGGL(k_adj_pi_fold<T>, dim3((pp.nk2 + 63) / 64, g.NY, pp.ni), dim3(64), 0, c->stream, g, (const T *)pihat, r1, pp.nk2);
+        DYN_LAUNCH_CHECK();
+        long long n = (long long)c->npts;
+        hipLaunchKernelGGL(k_zero_field<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, pihat, n);
+        DYN_LAUNCH_CHECK();
+    }
+    T *spec = (T *)pp.spec;
+    ThomasArgs ta{pp.coskx, pp.cosly, c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz,
+                  pp.ni, pp.nj, c->g.nz};
+    const int nkx = pp.ni / 2 + 1;
+    int kc = (int)(kLdsBudget / (2 * (size_t)pp.ni * sizeof(T))) * 2;
+    int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
+    kc = std::max(2, std::min(kc, want));
+    while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
+    size_t shx = 2 * (size_t)pp.ni * (kc / 2) * sizeof(T);
+    dim3 gx(g.NY, (pp.nk2 + kc - 1) / kc);
+    {
+        Timed _t(c, "adj_rfft_x", pts);
+        hipLaunchKernelGGL(k_div_rfft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)nullptr, (const T *)nullptr,
+                           (const T *)nullptr, spec, rx, pp.twx, pp.nk2, kc, step, (const T *)r1);
+    }
+    DYN_LAUNCH_CHECK();
+    const int pad = 1;
+    size_t shy = 2 * (size_t)pp.nj * (pp.nk2 + pad) * sizeof(T) + 8 + 16 * (size_t)pp.nk2 + 8;
+    if (pp.fused_y && shy <= pp.lds_max && max_butterflies(ry, pp.nk2, 1024) <= kMaxBY) {
+        Timed _t(c, "adj_ypass", pts);
+        hipLaunchKernelGGL((k_ypass<T, true>), dim3(nkx), dim3(1024), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
+        DYN_LAUNCH_CHECK();
+    } else {
+        int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
+        size_t sh2 = 4 * (size_t)pp.nj * kcy * sizeof(T);
+        dim3 gy(nkx, (pp.nk2 + kcy - 1) / kcy);
+        Timed _t(c, "adj_ypass", pts);
+        hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
+        DYN_LAUNCH_CHECK();
+        const int tg = 64;
+        size_t sht = ((tg * (2 * (size_t)pp.nk2 + 1) * sizeof(T) + 7) / 8) * 8 + 16 * (size_t)pp.nk2;
+        hipLaunchKernelGGL((k_thomas_staged<T, true>), dim3(nkx, (pp.nj + tg - 1) / tg), dim3(tg), sht, c->stream, spec, pp.nk2, ta);
+        DYN_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
+        DYN_LAUNCH_CHECK();
+    }
+    {
+        Timed _t(c, "adj_irfft_x", pts);
+        hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, r2, rx, pp.twx, pp.nk2, kc);
+    }
+    DYN_LAUNCH_CHECK();
+    {
+        Timed _t(c, "adj_div", (double)c->npts);
+        hipLaunchKernelGGL(k_adj_div<T>, dim3((g.NZ + 63) / 64, g.NY, g.NX), dim3(64), 0, c->stream, g, c->cols, (const T *)r2, uh, vh, wh,
+                           step);
+    }
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
 }  // namespace
 
 int pressure_init(Ctx *c) {
@@ -799,7 +991,10 @@ int pressure_init(Ctx *c) {
     if (pp.fused_y) {
         size_t need = 2 * (size_t)pp.nj * (pp.nk2 + 1) * sizeof(float) + 8 + 16 * (size_t)pp.nk2 + 8;
         if (need > pp.lds_max) pp.fused_y = false;
-        else DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        else {
+            DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+            DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        }
     }
     DYN_HIP(hipMalloc(&pp.ffac, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
     DYN_HIP(hipMalloc(&pp.coskx, sizeof(double) * pp.ni));
@@ -850,6 +1045,15 @@ int pressure_free(Ctx *c) {
 
 int launch_solve_pressure(Ctx *c, double step, bool halos) {
     return c->prec == DYN_FP32 ? solve_t<float>(c, step, halos) : solve_t<double>(c, step, halos);
+}
+
+// adjoint of solve_pressure(step) minus its periodic_uvw_boundaries: consumes (and zeroes) the adjoint of
+// PI and adds the adjoint of the divergence input to the forecast-velocity adjoints; r1, r2 are scratch
+// fields of the context's element type
+int launch_pressure_adjoint(Ctx *c, double step, void *pihat, void *uh, void *vh, void *wh, void *r1, void *r2) {
+    if (c->prec == DYN_FP32)
+        return solve_adjoint_t<float>(c, step, (float *)pihat, (float *)uh, (float *)vh, (float *)wh, (float *)r1, (float *)r2);
+    return solve_adjoint_t<double>(c, step, (double *)pihat, (double *)uh, (double *)vh, (double *)wh, (double *)r1, (double *)r2);
 }
 
 }  // namespace dyn
