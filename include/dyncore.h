@@ -149,6 +149,25 @@ int dyn_adjoint_backward(dyn_ctx *ctx, const double *seed_u, const double *seed_
                          double *grad_u, double *grad_v, double *grad_w, double *grad_th);
 int dyn_adjoint_release(dyn_ctx *ctx);
 
+/* ---- ensembles ----------------------------------------------------------------------------- */
+/* No counterpart in the reference (its ensemble driver runs members one after another on the CPU).
+ * An ensemble advances existing contexts of possibly different NX, NY (same precision and NZ, dry,
+ * NZ <= 64) together: every pass of the step is one persistent launch over all members, and
+ * dyn_ensemble_step replays a captured hipGraph.  Members must outlive the ensemble; call
+ * dyn_ensemble_synchronize before reading or writing a member's fields.  Per member the results are
+ * bit for bit those of dyn_step on that member alone. */
+typedef struct dyn_ensemble dyn_ensemble;
+int dyn_ensemble_create(dyn_ctx *const *members, int n, dyn_ensemble **out);
+int dyn_ensemble_destroy(dyn_ensemble *ens);
+int dyn_ensemble_synchronize(dyn_ensemble *ens);
+int dyn_ensemble_step(dyn_ensemble *ens, int nsteps);
+int dyn_ensemble_step_eager(dyn_ensemble *ens, int nsteps);
+/* launch shape {momentum rows per block, momentum march length, scalar rows, scalar march length} */
+int dyn_ensemble_set_shape(dyn_ensemble *ens, const int *shape);
+/* times every shape of the two stage kernels on the members' grids and keeps the fastest (written to
+ * shape_out[4] when not NULL); the model state is unchanged */
+int dyn_ensemble_autotune(dyn_ensemble *ens, int *shape_out);
+
 #ifdef __cplusplus
 }
 #endif

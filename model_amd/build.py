@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libdyncore.so")
 REFAPI_LIB = os.path.join(OUT, "libdyncore_refapi.so")
 
-HIP_SOURCES = ["dyn_capi.hip", "dyn_stencil.hip", "dyn_pressure.hip", "dyn_fused.hip", "dyn_adjoint.hip"]
+HIP_SOURCES = ["dyn_capi.hip", "dyn_stencil.hip", "dyn_pressure.hip", "dyn_fused.hip", "dyn_adjoint.hip", "dyn_ensemble.hip"]
 REFAPI_SOURCES = ["refapi/refapi.cpp", "refapi/driver_serial.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

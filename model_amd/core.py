@@ -28,6 +28,8 @@ EXPORTS = [
     "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner", "dyn_micro_advance_outer",
     "dyn_step_reference", "dyn_step", "dyn_step_fused", "dyn_algorithmic_bytes_per_point", "dyn_timing", "dyn_timing_report",
     "dyn_stream", "dyn_tune_set", "dyn_tune_get", "dyn_adjoint_record", "dyn_adjoint_backward", "dyn_adjoint_release",
+    "dyn_ensemble_create", "dyn_ensemble_destroy", "dyn_ensemble_synchronize", "dyn_ensemble_step", "dyn_ensemble_step_eager",
+    "dyn_ensemble_set_shape", "dyn_ensemble_autotune",
 ]
 
 
@@ -85,6 +87,13 @@ def load():
     L.dyn_adjoint_record.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.dyn_adjoint_backward.argtypes = [ctypes.c_void_p] + [_dp] * 8
     L.dyn_adjoint_release.argtypes = [ctypes.c_void_p]
+    L.dyn_ensemble_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    for fn in ("dyn_ensemble_destroy", "dyn_ensemble_synchronize"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p]
+    for fn in ("dyn_ensemble_step", "dyn_ensemble_step_eager"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dyn_ensemble_set_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    L.dyn_ensemble_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     L.dyn_stream.restype = ctypes.c_void_p
     _lib = L
     return L
@@ -183,3 +192,41 @@ class DynCore:
             _check(fn(h, *a))
 
         return call
+
+
+class Ensemble:
+    """Members of possibly different horizontal sizes advanced together (dyn_ensemble_*)."""
+
+    def __init__(self, members):
+        self.L = load()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _check(self.L.dyn_ensemble_create(arr, len(self.members), ctypes.byref(h)))
+        self.h = h
+
+    def step(self, n=1, graph=True):
+        _check((self.L.dyn_ensemble_step if graph else self.L.dyn_ensemble_step_eager)(self.h, n))
+
+    def sync(self):
+        _check(self.L.dyn_ensemble_synchronize(self.h))
+
+    def set_shape(self, shape):
+        arr = (ctypes.c_int * 4)(*shape)
+        _check(self.L.dyn_ensemble_set_shape(self.h, arr))
+
+    def autotune(self):
+        arr = (ctypes.c_int * 4)()
+        _check(self.L.dyn_ensemble_autotune(self.h, arr))
+        return tuple(arr)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.dyn_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

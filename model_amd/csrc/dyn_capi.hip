@@ -803,3 +803,8 @@ int step_with_hook(Ctx *c, StageHook hook, void *arg) {
     return fused_supported(c) ? fused_step(ctx, hook, arg) : reference_step(ctx, hook, arg);
 }
 }  // namespace dyn
+
+namespace dyn {
+// the damping coefficients of rayleigh_damping for `level` (host computation and upload; not in a capture)
+int rayleigh_table(Ctx *c, int level) { return ensure_rayleigh_table(c, level); }
+}  // namespace dyn

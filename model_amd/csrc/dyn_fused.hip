@@ -11,6 +11,8 @@
 //     bottom faces of the neighbours.
 // Every face and tendency uses the same expression tree as the per-operator kernels, so the fused and
 // unfused paths agree to the last bit.
+#include <algorithm>
+
 #include "dyn_internal.hpp"
 
 namespace dyn {
@@ -81,18 +83,17 @@ struct MomLds {
     static constexpr size_t bytes = (uv + w + nf) * sizeof(T);
 };
 
+// one tile (TJ rows x ti columns x all levels) of the momentum stage; every thread of the block calls it
 template <typename T, int TJ, bool MOIST>
-__global__ void __launch_bounds__(KP *TJ) k_adv_momentum(Geo g, Cols c, StageArgs a, T step) {
+__device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, const StageArgs &a, T step, int t, unsigned char *smem_raw) {
     using L = MomLds<T, TJ>;
     constexpr int R = L::R, RW = L::RW;
     constexpr int HU = (R + TJ - 1) / TJ, HW = (RW + TJ - 1) / TJ;  // plane rows per wavefront
-    extern __shared__ unsigned char smem_raw[];
     T *uvp = (T *)smem_raw;
     T *wpl = uvp + L::uv;
     T *nface = wpl + L::w;
 
     const int k = threadIdx.x, r = threadIdx.y;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int j0 = a.jl + (t % a.nbj) * TJ;
     const int i0 = a.il + (t / a.nbj) * a.ti;
     const int i1 = min(i0 + a.ti, a.ih);
@@ -346,16 +347,14 @@ struct ScalLds {
 };
 
 template <typename T, int TJ, bool MOIST>
-__global__ void __launch_bounds__(KP *TJ) k_adv_scalars(Geo g, Cols c, StageArgs a, T step) {
+__device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, const StageArgs &a, T step, int t, unsigned char *smem_raw) {
     using L = ScalLds<T, TJ, MOIST>;
     constexpr int R = L::R, NS = L::NS, NFS = L::NFS;
     constexpr int HS = (R + TJ - 1) / TJ;
-    extern __shared__ unsigned char smem_raw[];
     T *spl = (T *)smem_raw;
     T *nface = spl + L::s;
 
     const int k = threadIdx.x, r = threadIdx.y;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int j0 = a.jl + (t % a.nbj) * TJ;
     const int i0 = a.il + (t / a.nbj) * a.ti;
     const int i1 = min(i0 + a.ti, a.ih);
@@ -614,6 +613,18 @@ __global__ void __launch_bounds__(KP *TJ) k_adv_scalars(Geo g, Cols c, StageArgs
     }
 }
 
+template <typename T, int TJ, bool MOIST>
+__global__ void __launch_bounds__(KP *TJ) k_adv_momentum(Geo g, Cols c, StageArgs a, T step) {
+    extern __shared__ unsigned char smem_raw[];
+    adv_momentum_tile<T, TJ, MOIST>(g, c, a, step, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
+}
+
+template <typename T, int TJ, bool MOIST>
+__global__ void __launch_bounds__(KP *TJ) k_adv_scalars(Geo g, Cols c, StageArgs a, T step) {
+    extern __shared__ unsigned char smem_raw[];
+    adv_scalars_tile<T, TJ, MOIST>(g, c, a, step, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
+}
+
 // zero_moisture outside the advected box (the fused kernel clips the points it writes)
 template <typename T>
 __global__ void k_zero_moisture_shell(Geo g, Cols c, T *__restrict__ QVP, T *__restrict__ QCP, T *__restrict__ QRP,
@@ -733,10 +744,9 @@ __device__ __forceinline__ void write_col(const Geo &g, T *A, int isrc, int j, i
 
 // PGF (solver.cpp:123-135) + sponge/periodic/lid for UP, VP and WP
 template <typename T>
-__global__ void __launch_bounds__(256) k_post_uvw(Geo g, Cols c, PostArgs a, T step, bool pgf) {
+__device__ __forceinline__ void post_uvw_col(const Geo &g, const Cols &c, const PostArgs &a, T step, bool pgf, int jb, int isrc) {
     const int k = threadIdx.x;
-    const int j = blockIdx.x * blockDim.y + threadIdx.y;
-    const int isrc = blockIdx.y + 3;
+    const int j = jb * blockDim.y + threadIdx.y;
     if (j >= g.NY) return;
     const int NZ = g.NZ;
     const int kc = k < NZ ? k : NZ - 1;
@@ -773,10 +783,9 @@ __global__ void __launch_bounds__(256) k_post_uvw(Geo g, Cols c, PostArgs a, T s
 // columns a sponge row, a ghost row or the periodic copy touches are rewritten whole, the others get the
 // two lid levels
 template <typename T, bool MOIST>
-__global__ void __launch_bounds__(256) k_post_scalars(Geo g, Cols c, PostArgs a, bool clip) {
+__device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, const PostArgs &a, bool clip, int jb, int isrc) {
     const int k = threadIdx.x;
-    const int j = blockIdx.x * blockDim.y + threadIdx.y;
-    const int isrc = blockIdx.y + 3;
+    const int j = jb * blockDim.y + threadIdx.y;
     if (j >= g.NY) return;
     const int NZ = g.NZ;
     const int kc = k < NZ ? k : NZ - 1;
@@ -815,6 +824,99 @@ __global__ void __launch_bounds__(256) k_post_scalars(Geo g, Cols c, PostArgs a,
             }
             write_col(g, A, isrc, j, k, fin, x);
         }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_post_uvw(Geo g, Cols c, PostArgs a, T step, bool pgf) {
+    post_uvw_col<T>(g, c, a, step, pgf, blockIdx.x, blockIdx.y + 3);
+}
+template <typename T, bool MOIST>
+__global__ void __launch_bounds__(256) k_post_scalars(Geo g, Cols c, PostArgs a, bool clip) {
+    post_scalars_col<T, MOIST>(g, c, a, clip, blockIdx.x, blockIdx.y + 3);
+}
+
+// ---- batched over the members of an ensemble (persistent blocks walking the concatenated work) --------
+__device__ __forceinline__ StageArgs ens_stage_args(const EnsMember &e, const EnsRoles &ro, int ti, int nbj) {
+    StageArgs a;
+    for (int f = 0; f < 7; f++) {
+        a.cur[f] = e.buf[f][ro.r[f][DYN_ROLE_CUR]];
+        a.mprev[f] = e.buf[f][ro.r[f][DYN_ROLE_M]];
+        a.out[f] = e.buf[f][ro.r[f][DYN_ROLE_P]];
+    }
+    for (int f = 0; f < 5; f++) a.bar[f] = e.bar[f];
+    a.vt = e.vt;
+    a.il = 3;
+    a.ih = e.g.NX - 3;
+    a.jl = 3;
+    a.jh = e.g.NY - 3;
+    a.ti = ti;
+    a.nbj = nbj;
+    return a;
+}
+__device__ __forceinline__ PostArgs ens_post_args(const EnsMember &e, const EnsRoles &ro) {
+    PostArgs a;
+    for (int f = 0; f < 7; f++) a.f[f] = e.buf[f][ro.r[f][DYN_ROLE_P]];
+    a.pi = e.pi;
+    a.qb = e.bar[4];
+    a.cs = e.sponge_s;
+    a.cn = e.sponge_n;
+    a.js = e.js;
+    a.jn = e.jn;
+    a.zl = 1;
+    a.zh = e.g.NX - 1;
+    a.zjl = 1;
+    a.zjh = e.g.NY - 1;
+    return a;
+}
+
+template <typename T, int TJ, bool MOIST>
+__global__ void __launch_bounds__(KP *TJ) k_ens_momentum(const EnsMember *mem, int nmem, EnsRoles ro, T step, int ti, int total) {
+    extern __shared__ unsigned char smem_raw[];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::mom0)];
+        const StageArgs a = ens_stage_args(e, ro, ti, e.nbj_mom);
+        __syncthreads();  // the previous tile's LDS reads are done
+        adv_momentum_tile<T, TJ, MOIST>(e.g, e.c, a, step, item - e.mom0, smem_raw);
+    }
+}
+
+template <typename T, int TJ, bool MOIST>
+__global__ void __launch_bounds__(KP *TJ) k_ens_scalars(const EnsMember *mem, int nmem, EnsRoles ro, T step, int ti, int total) {
+    extern __shared__ unsigned char smem_raw[];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::scal0)];
+        const StageArgs a = ens_stage_args(e, ro, ti, e.nbj_scal);
+        __syncthreads();
+        adv_scalars_tile<T, TJ, MOIST>(e.g, e.c, a, step, item - e.scal0, smem_raw);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ens_post(const EnsMember *mem, int nmem, EnsRoles ro, T step, bool pgf, bool scalars, int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::post0)];
+        const int local = item - e.post0, nb = (e.g.NY + 3) / 4;
+        const PostArgs a = ens_post_args(e, ro);
+        post_uvw_col<T>(e.g, e.c, a, step, pgf, local % nb, local / nb + 3);
+        if (scalars) post_scalars_col<T, false>(e.g, e.c, a, false, local % nb, local / nb + 3);
+    }
+}
+
+// advance_outer's memcpy(M <- P) for u, v, w, theta of every member; work items are 1024-element chunks
+template <typename T>
+__global__ void __launch_bounds__(256) k_ens_copy(const EnsMember *mem, int nmem, EnsRoles ro, int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::cp0)];
+        const long long n = (long long)e.g.NX * e.g.NYNZ;
+        const long long per = (n + 1023) / 1024;
+        const int local = item - e.cp0;
+        const int fi = local / (int)per;
+        const int f = fi == 0 ? DYN_U : fi == 1 ? DYN_V : fi == 2 ? DYN_W : DYN_TH;
+        T *dst = (T *)e.buf[f][ro.r[f][DYN_ROLE_M]];
+        const T *src = (const T *)e.buf[f][ro.r[f][DYN_ROLE_P]];
+        const long long base = (long long)(local % (int)per) * 1024;
+        for (long long p = base + threadIdx.x; p < base + 1024 && p < n; p += blockDim.x) dst[p] = src[p];
     }
 }
 
@@ -924,6 +1026,99 @@ int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip) {
 int launch_zero_moisture_shell(Ctx *c, int il, int ih, int jl, int jh) {
     if (c->prec == DYN_FP32) return shell_t<float>(c, il, ih, jl, jh);
     return shell_t<double>(c, il, ih, jl, jh);
+}
+
+
+// ---- ensemble launchers ----------------------------------------------------------------------------
+int cu_count() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) n = 256;
+    }
+    return n;
+}
+// persistent grid: as many blocks as can be resident at once, never more than there is work
+int persistent_grid(const void *kernel, int threads, size_t lds, int total) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    return std::max(1, std::min(total, per_cu * cu_count()));
+}
+
+namespace {
+template <typename T, int TJ>
+int ens_momentum_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, T step, int ti, int total) {
+    const size_t sh = MomLds<T, TJ>::bytes;
+    const void *k = (const void *)k_ens_momentum<T, TJ, false>;
+    const int grid = persistent_grid(k, KP * TJ, sh, total);
+    Timed _t(c0, "ens_momentum", 0.0);
+    hipLaunchKernelGGL((k_ens_momentum<T, TJ, false>), dim3(grid), dim3(KP, TJ), sh, c0->stream, d, nmem, ro, step, ti, total);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+template <typename T, int TJ>
+int ens_scalars_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, T step, int ti, int total) {
+    const size_t sh = ScalLds<T, TJ, false>::bytes;
+    const void *k = (const void *)k_ens_scalars<T, TJ, false>;
+    const int grid = persistent_grid(k, KP * TJ, sh, total);
+    Timed _t(c0, "ens_scalars", 0.0);
+    hipLaunchKernelGGL((k_ens_scalars<T, TJ, false>), dim3(grid), dim3(KP, TJ), sh, c0->stream, d, nmem, ro, step, ti, total);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+template <typename T>
+int ens_stage_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, const int *shape, int tot_mom, int tot_scal) {
+    int rc;
+    if (shape[0] == 2) rc = ens_momentum_t<T, 2>(c0, d, nmem, ro, T(step), shape[1], tot_mom);
+    else if (shape[0] == 4) rc = ens_momentum_t<T, 4>(c0, d, nmem, ro, T(step), shape[1], tot_mom);
+    else rc = ens_momentum_t<T, 8>(c0, d, nmem, ro, T(step), shape[1], tot_mom);
+    if (rc) return rc;
+    if (shape[2] == 2) return ens_scalars_t<T, 2>(c0, d, nmem, ro, T(step), shape[3], tot_scal);
+    if (shape[2] == 4) return ens_scalars_t<T, 4>(c0, d, nmem, ro, T(step), shape[3], tot_scal);
+    return ens_scalars_t<T, 8>(c0, d, nmem, ro, T(step), shape[3], tot_scal);
+}
+template <typename T>
+int ens_post_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, bool pgf, int total) {
+    const int grid = persistent_grid((const void *)k_ens_post<T>, 256, 0, total);
+    Timed _t(c0, "ens_post", 0.0);
+    hipLaunchKernelGGL(k_ens_post<T>, dim3(grid), dim3(KP, 4), 0, c0->stream, d, nmem, ro, T(step), pgf, true, total);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+template <typename T>
+int ens_copy_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, int total) {
+    const int grid = persistent_grid((const void *)k_ens_copy<T>, 256, 0, total);
+    Timed _t(c0, "ens_advance_copy", 0.0);
+    hipLaunchKernelGGL(k_ens_copy<T>, dim3(grid), dim3(256), 0, c0->stream, d, nmem, ro, total);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+template <typename T, int TJ>
+int ens_lds_limits() {
+    DYN_HIP(hipFuncSetAttribute((const void *)k_ens_momentum<T, TJ, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)MomLds<T, TJ>::bytes));
+    DYN_HIP(hipFuncSetAttribute((const void *)k_ens_scalars<T, TJ, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ScalLds<T, TJ, false>::bytes));
+    return DYN_OK;
+}
+}  // namespace
+
+int ens_prepare(bool fp32) {
+    int rc = fp32 ? ens_lds_limits<float, 2>() : ens_lds_limits<double, 2>();
+    if (!rc) rc = fp32 ? ens_lds_limits<float, 4>() : ens_lds_limits<double, 4>();
+    if (!rc) rc = fp32 ? ens_lds_limits<float, 8>() : ens_lds_limits<double, 8>();
+    return rc;
+}
+// the stage advection of every member (one momentum and one scalar launch); shape = {mom TJ, mom TI, scal TJ, scal TI}
+int launch_ens_stage(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, const int *shape, int tot_mom, int tot_scal) {
+    return c0->prec == DYN_FP32 ? ens_stage_t<float>(c0, d, nmem, ro, step, shape, tot_mom, tot_scal)
+                                : ens_stage_t<double>(c0, d, nmem, ro, step, shape, tot_mom, tot_scal);
+}
+int launch_ens_post(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, bool pgf, int total) {
+    return c0->prec == DYN_FP32 ? ens_post_t<float>(c0, d, nmem, ro, step, pgf, total) : ens_post_t<double>(c0, d, nmem, ro, step, pgf, total);
+}
+int launch_ens_copy(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, int total) {
+    return c0->prec == DYN_FP32 ? ens_copy_t<float>(c0, d, nmem, ro, total) : ens_copy_t<double>(c0, d, nmem, ro, total);
 }
 
 }  // namespace dyn

@@ -53,6 +53,13 @@ struct Cols {
     const double *tb, *tbv, *tbw, *qb, *pib, *f;
 };
 
+// Mixed-radix plan of one transform length (radix 4/2/3/5 passes).
+constexpr int kMaxRad = 8;
+struct Radix {
+    int r[kMaxRad];
+    int n, nr;
+};
+
 // Pressure-solver plan (serial periodic branch: ni = NX-6, nj = NY, levels 1..NZ-2).
 struct PressurePlan {
     int ni = 0, nj = 0, nk2 = 0;      // transform sizes and active level count
@@ -73,6 +80,37 @@ struct TimerRec {
     hipEvent_t a, b;
     double units;  // gridpoints the launch processes
 };
+
+// One member of an ensemble as the batched kernels see it: a device-resident table entry, fixed for the
+// ensemble's lifetime.  The three buffers of every prognostic are listed in creation order; which one
+// plays M, current or forecast is passed to each launch as EnsRoles (all members rotate in step).
+struct EnsMember {
+    Geo g;
+    Cols c;
+    void *buf[7][3];
+    const void *bar[6];
+    void *pi, *vt;
+    const double *sponge_n, *sponge_s;
+    int js, jn;
+    // pressure plan
+    void *spec;
+    const double *twx, *twy, *ffac, *coskx, *cosly, *acoef;
+    Radix rx, ry;
+    int ni, nj, nk2, kcx;
+    double dx, dz;
+    // first work item of this member in each batched launch
+    int mom0, scal0, post0, px0, py0, cp0;
+    int nbj_mom, nbj_scal;
+};
+struct EnsRoles {
+    unsigned char r[7][3];  // buffer index of (field, role)
+};
+// member owning work item `item` of a batched launch whose per-member offsets are the field `first`
+__device__ __forceinline__ int ens_member(const EnsMember *mem, int nmem, int item, int EnsMember::*first) {
+    int m = 0;
+    while (m + 1 < nmem && mem[m + 1].*first <= item) m++;
+    return m;
+}
 
 struct AdjState;  // trajectory and adjoint buffers of the hand-written adjoint (dyn_adjoint.hip)
 
@@ -135,6 +173,7 @@ int launch_bc(Ctx *c, bool micro);
 int launch_periodic(Ctx *c, void *const *fields, int nfields);
 int launch_kessler(Ctx *c, int il, int ih, int jl, int jh);
 int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level);
+int rayleigh_table(Ctx *c, int level);
 int pressure_init(Ctx *c);
 int pressure_free(Ctx *c);
 int launch_solve_pressure(Ctx *c, double step, bool halos = true);
@@ -145,6 +184,16 @@ void adjoint_release(Ctx *c);
 typedef int (*StageHook)(Ctx *c, int stage, void *arg);
 int step_with_hook(Ctx *c, StageHook hook, void *arg);
 bool fused_supported(const Ctx *c);
+// ensembles: batched launches over a device table of members (dyn_fused.hip, dyn_pressure.hip)
+int cu_count();
+int persistent_grid(const void *kernel, int threads, size_t lds, int total);
+int ens_prepare(bool fp32);
+int launch_ens_stage(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, const int *shape, int tot_mom, int tot_scal);
+int launch_ens_post(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, bool pgf, int total);
+int launch_ens_copy(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, int total);
+bool pressure_ens_fill(Ctx *c, EnsMember &e, size_t &lds_x, size_t &lds_y);
+int launch_pressure_ens(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, int tot_px, int tot_py, size_t lds_x,
+                        size_t lds_y);
 // launch shapes of the stage kernels (DYN_TUNE_*): validity and the value in force
 bool fused_shape_ok(const Ctx *c, int param, int value);
 int fused_shape(const Ctx *c, int param);

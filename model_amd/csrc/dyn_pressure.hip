@@ -18,11 +18,6 @@
 namespace dyn {
 namespace {
 
-constexpr int kMaxRad = 8;
-struct Radix {
-    int r[kMaxRad];
-    int n, nr;
-};
 
 constexpr int kThreads = 256;
 constexpr size_t kLdsBudget = 64 * 1024;
@@ -416,14 +411,13 @@ constexpr int kMaxBY = 4;   // and in the fused y pass
 // With src != nullptr the transform reads a ready real field src(i+3, j, k) instead of the divergence
 // (the transposed solve of the adjoint).
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T *__restrict__ UP, const T *__restrict__ VP,
-                                                         const T *__restrict__ WP, T *__restrict__ spec, Radix rad, const double *tw,
-                                                         int nk2, int kc, double step, const T *__restrict__ src) {
-    extern __shared__ unsigned char smem[];
+__device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, const T *__restrict__ UP, const T *__restrict__ VP,
+                                                 const T *__restrict__ WP, T *__restrict__ spec, const Radix &rad, const double *tw,
+                                                 int nk2, int kc, double step, const T *__restrict__ src, int j, int chunk,
+                                                 unsigned char *smem) {
     const int n = rad.n, P = kc / 2, nkx = n / 2 + 1;
     T *re = (T *)smem, *im = re + n * P;
-    const int j = blockIdx.x;
-    const int kk0 = blockIdx.y * kc;
+    const int kk0 = chunk * kc;
     const int jj = j == 0 ? 1 : (j == g.NY - 1 ? g.NY - 2 : j);  // mirrored rows, pressure.cpp:1408-1415
     const double sdt = step * g.dt;
     for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
@@ -466,12 +460,11 @@ __global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T 
 
 // Q3: backward real DFT along i -> PI (pressure.cpp:878-892) with the ghost levels and periodic halos
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_irfft_x_out(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
-                                                          const double *tw, int nk2, int kc) {
-    extern __shared__ unsigned char smem[];
+__device__ __forceinline__ void irfft_x_out_block(const Geo &g, const T *__restrict__ spec, T *__restrict__ PI, const Radix &rad,
+                                                  const double *tw, int nk2, int kc, int j, int chunk, unsigned char *smem) {
     const int n = rad.n, P = kc / 2, half = n / 2;
     T *re = (T *)smem, *im = re + n * P;
-    const int j = blockIdx.x, kk0 = blockIdx.y * kc;
+    const int kk0 = chunk * kc;
     // rebuild Z = A + iB for m in [0, n) from the half spectra of the two levels of each pair
     for (int idx = threadIdx.x; idx < n * P; idx += blockDim.x) {
         int pp = idx % P, m = idx / P;
@@ -627,14 +620,12 @@ __device__ void thomas_column_t(const ThomasArgs &ta, int kx, int ky, T *x, int 
 
 // Q2 (fused): forward DFT along j, FP64 tridiagonal solve, backward DFT along j for one kx, with the
 // whole (j, k) slab of that kx resident in LDS
-template <typename T, bool TR = false>
-__global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, Radix rady, const double *twy, ThomasArgs ta,
-                                               int pad) {
-    extern __shared__ unsigned char smem[];
+template <typename T, bool TR>
+__device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const Radix &rady, const double *twy, const ThomasArgs &ta,
+                                            int pad, int kx, unsigned char *smem) {
     const int nj = rady.n, P2 = nk2 + pad;
     T *re = (T *)smem, *im = re + nj * P2;
     double *mean = (double *)(re + 2 * nj * P2);  // 2*nj*P2 elements of T: 8-byte aligned
-    const int kx = blockIdx.x;
     const long long base = (long long)kx * nj * nk2 * 2;
     for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
         int kk = idx % nk2, ky = idx / nk2;
@@ -655,6 +646,68 @@ __global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, R
         int kk = idx % nk2, ky = idx / nk2;
         spec[base + 2 * (long long)idx] = re[ky * P2 + kk];
         spec[base + 2 * (long long)idx + 1] = im[ky * P2 + kk];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T *__restrict__ UP, const T *__restrict__ VP,
+                                                         const T *__restrict__ WP, T *__restrict__ spec, Radix rad, const double *tw,
+                                                         int nk2, int kc, double step, const T *__restrict__ src) {
+    extern __shared__ unsigned char smem[];
+    div_rfft_x_block<T>(g, c, UP, VP, WP, spec, rad, tw, nk2, kc, step, src, blockIdx.x, blockIdx.y, smem);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_irfft_x_out(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
+                                                          const double *tw, int nk2, int kc) {
+    extern __shared__ unsigned char smem[];
+    irfft_x_out_block<T>(g, spec, PI, rad, tw, nk2, kc, blockIdx.x, blockIdx.y, smem);
+}
+
+template <typename T, bool TR = false>
+__global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, Radix rady, const double *twy, ThomasArgs ta, int pad) {
+    extern __shared__ unsigned char smem[];
+    ypass_block<T, TR>(spec, nk2, rady, twy, ta, pad, blockIdx.x, smem);
+}
+
+// ---- batched over the members of an ensemble: one launch per pass, blocks walk a work list that
+// concatenates the members' (row, level chunk) or (kx) items -------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_ens_div_rfft_x(const EnsMember *mem, int nmem, EnsRoles roles, int total, double step) {
+    extern __shared__ unsigned char smem[];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int m = ens_member(mem, nmem, item, &EnsMember::px0);
+        const EnsMember &e = mem[m];
+        const int local = item - e.px0, nch = (e.nk2 + e.kcx - 1) / e.kcx;
+        __syncthreads();
+        div_rfft_x_block<T>(e.g, e.c, (const T *)e.buf[DYN_U][roles.r[DYN_U][DYN_ROLE_P]], (const T *)e.buf[DYN_V][roles.r[DYN_V][DYN_ROLE_P]],
+                            (const T *)e.buf[DYN_W][roles.r[DYN_W][DYN_ROLE_P]], (T *)e.spec, e.rx, e.twx, e.nk2, e.kcx, step,
+                            (const T *)nullptr, local / nch, local % nch, smem);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_ens_irfft_x_out(const EnsMember *mem, int nmem, int total) {
+    extern __shared__ unsigned char smem[];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int m = ens_member(mem, nmem, item, &EnsMember::px0);
+        const EnsMember &e = mem[m];
+        const int local = item - e.px0, nch = (e.nk2 + e.kcx - 1) / e.kcx;
+        __syncthreads();
+        irfft_x_out_block<T>(e.g, (const T *)e.spec, (T *)e.pi, e.rx, e.twx, e.nk2, e.kcx, local / nch, local % nch, smem);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_ens_ypass(const EnsMember *mem, int nmem, int total) {
+    extern __shared__ unsigned char smem[];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int m = ens_member(mem, nmem, item, &EnsMember::py0);
+        const EnsMember &e = mem[m];
+        ThomasArgs ta{e.coskx, e.cosly, e.c.rhou, e.c.rhow, e.c.mu, e.c.mw, e.acoef, e.ffac, e.dx, e.dz, e.ni, e.nj, e.g.NZ};
+        __syncthreads();
+        ypass_block<T, false>((T *)e.spec, e.nk2, e.ry, e.twy, ta, 1, item - e.py0, smem);
     }
 }
 
@@ -1045,6 +1098,71 @@ int pressure_free(Ctx *c) {
 
 int launch_solve_pressure(Ctx *c, double step, bool halos) {
     return c->prec == DYN_FP32 ? solve_t<float>(c, step, halos) : solve_t<double>(c, step, halos);
+}
+
+
+namespace {
+template <typename T>
+int level_pairs(const PressurePlan &pp, const Geo &g, const Radix &rx) {
+    int kc = (int)(kLdsBudget / (2 * (size_t)pp.ni * sizeof(T))) * 2;
+    int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
+    kc = std::max(2, std::min(kc, want));
+    while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
+    return kc;
+}
+}  // namespace
+
+// the pressure part of an ensemble member's table entry; false when the member cannot join the batched
+// solve (odd ni, or a (j, k) slab too large for the LDS-resident j pass)
+bool pressure_ens_fill(Ctx *c, EnsMember &e, size_t &lds_x, size_t &lds_y) {
+    PressurePlan &pp = c->pp;
+    if (pp.ni % 2 != 0 || !pp.fused_y || c->prec != DYN_FP32) return false;
+    if (!factorize(pp.ni, e.rx) || !factorize(pp.nj, e.ry)) return false;
+    e.kcx = level_pairs<float>(pp, c->geo, e.rx);
+    e.ni = pp.ni;
+    e.nj = pp.nj;
+    e.nk2 = pp.nk2;
+    e.spec = pp.spec;
+    e.twx = pp.twx;
+    e.twy = pp.twy;
+    e.ffac = pp.ffac;
+    e.coskx = pp.coskx;
+    e.cosly = pp.cosly;
+    e.acoef = pp.acoef;
+    e.dx = c->g.dx;
+    e.dz = c->g.dz;
+    lds_x = 2 * (size_t)pp.ni * (e.kcx / 2) * sizeof(float);
+    lds_y = 2 * (size_t)pp.nj * (pp.nk2 + 1) * sizeof(float) + 8 + 16 * (size_t)pp.nk2 + 8;
+    return lds_y <= pp.lds_max && max_butterflies(e.ry, pp.nk2, 1024) <= kMaxBY;
+}
+
+// solve_pressure(step) of every member without the uvw halo copy (as in the fused step): three launches
+int launch_pressure_ens(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, int tot_px, int tot_py, size_t lds_x,
+                        size_t lds_y) {
+    static size_t lds_y_set = 0;
+    if (lds_y > lds_y_set) {
+        DYN_HIP(hipFuncSetAttribute((const void *)k_ens_ypass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_y));
+        lds_y_set = lds_y;
+    }
+    {
+        const int grid = persistent_grid((const void *)k_ens_div_rfft_x<float>, kThreads, lds_x, tot_px);
+        Timed _t(c0, "ens_p1_div_rfft_x", 0.0);
+        hipLaunchKernelGGL(k_ens_div_rfft_x<float>, dim3(grid), dim3(kThreads), lds_x, c0->stream, d, nmem, ro, tot_px, step);
+        DYN_LAUNCH_CHECK();
+    }
+    {
+        const int grid = persistent_grid((const void *)k_ens_ypass<float>, 1024, lds_y, tot_py);
+        Timed _t(c0, "ens_p2_ypass", 0.0);
+        hipLaunchKernelGGL(k_ens_ypass<float>, dim3(grid), dim3(1024), lds_y, c0->stream, d, nmem, tot_py);
+        DYN_LAUNCH_CHECK();
+    }
+    {
+        const int grid = persistent_grid((const void *)k_ens_irfft_x_out<float>, kThreads, lds_x, tot_px);
+        Timed _t(c0, "ens_p5_irfft_x_out", 0.0);
+        hipLaunchKernelGGL(k_ens_irfft_x_out<float>, dim3(grid), dim3(kThreads), lds_x, c0->stream, d, nmem, tot_px);
+        DYN_LAUNCH_CHECK();
+    }
+    return DYN_OK;
 }
 
 // adjoint of solve_pressure(step) minus its periodic_uvw_boundaries: consumes (and zeroes) the adjoint of
