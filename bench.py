@@ -11,7 +11,16 @@ plain form is measured beside it.  Gridpoints per step = 256*256*64 (halo column
 
 The roofline object describes the dominant kernel of the FP32 step: its algorithmic HBM bytes per
 launch (field passes x element size x gridpoints processed) divided by its mean launch time, measured
-with HIP events on the context's own stream over an instrumented run of the same step sequence.
+with HIP events on the context's own stream over an instrumented run of the same step sequence.  Its
+`traffic` (measured HBM bytes per launch) comes from the rocprofv3 counter summary committed under
+profiles/ when one exists for that kernel.
+
+Two more measurements ride along (skip them with --no-extras):
+  ensemble  32 dry FP32 vortex members, horizontal sizes drawn from {96,128,160,192}^2, nz=40, advanced
+            by one persistent launch per pass from a hipGraph (launch shape autotuned), beside the same
+            members stepped one after another;
+  adjoint   the hand-written adjoint of the dry FP64 step on the 192x192x40 vortex-in-shear case, 360
+            steps with a snapshot every 20: backward time / forward time.
 """
 import argparse
 import json
@@ -82,9 +91,94 @@ def roofline(rows, elt, moist):
     bytes_per_launch = passes * elt * per_launch_units
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 5),
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(name), "avg_launch_ms": round(avg_ms, 5),
             "algorithmic_bytes_per_launch": int(bytes_per_launch), "field_passes_per_point": passes,
             "points_per_launch": int(per_launch_units)}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_supercell256.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    for name, row in data.get("kernels", {}).items():
+        if name.startswith(KERNEL_SYMBOL.get(kernel, kernel)):
+            return int(row["bytes"])
+    return None
+
+
+# timing-record name -> kernel symbol prefix in the rocprofv3 summaries
+KERNEL_SYMBOL = {"adv_scalars": "k_adv_scalars", "adv_momentum": "k_adv_momentum", "p2_ypass": "k_ypass",
+                 "p1_div_rfft_x": "k_div_rfft_x", "p5_irfft_x_out": "k_irfft_x_out", "post_uvw": "k_post_uvw",
+                 "post_scalars": "k_post_scalars"}
+
+
+def ensemble_bench(steps, warmup):
+    import numpy as np
+    from model_amd import configs
+    from model_amd.core import DYN_FP32, DynCore, Ensemble
+    rng = np.random.Generator(np.random.PCG64(20261202 + 3))
+    sizes = rng.choice([96, 128, 160, 192], size=(32, 2))
+    cases = [configs.vortex_in_shear(nx=int(a), ny=int(b), nz=40, seed_offset=100 + m) for m, (a, b) in enumerate(sizes)]
+    members = [DynCore(c, precision=DYN_FP32) for c in cases]
+    pts = sum(c.interior_points for c in cases)
+    ens = Ensemble(members)
+    shape = ens.autotune()
+    ens.step(warmup)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    t_ens = time.perf_counter() - t0
+    for m in members:  # each member alone, its own graph, one after another
+        m.step(warmup)
+    for m in members:
+        m.sync()
+    t0 = time.perf_counter()
+    for m in members:
+        m.step(steps)
+    for m in members:
+        m.sync()
+    t_seq = time.perf_counter() - t0
+    ens.close()
+    for m in members:
+        m.close()
+    return {"members": len(cases), "sizes": [[int(a), int(b)] for a, b in sizes], "nz": 40, "dtype": "f32",
+            "gridpoints_per_step": pts, "steps": steps, "ms_per_step": round(t_ens * 1e3 / steps, 4),
+            "value": round(pts * steps / t_ens, 1), "unit": "gridpoint-steps/s",
+            "members_one_by_one": round(pts * steps / t_seq, 1), "batched_over_one_by_one": round(t_seq / t_ens, 3),
+            "shape": {"momentum_tj": shape[0], "momentum_ti": shape[1], "scalar_tj": shape[2], "scalar_ti": shape[3]}}
+
+
+def adjoint_bench(steps=360, every=20):
+    from model_amd import configs
+    from model_amd.adjoint import backward, record, set_initial, vorticity_weights
+    from model_amd.core import DynCore
+    c = configs.vortex_in_shear()
+    core = DynCore(c)
+    x0 = [c.fields["u"], c.fields["v"], c.fields["w"], c.fields["th"]]
+    Wu, Wv = vorticity_weights(c)
+    core.step(4)
+    core.sync()
+    set_initial(core, x0)
+    t0 = time.perf_counter()
+    core.step(steps)
+    core.sync()
+    t_fwd = time.perf_counter() - t0
+    set_initial(core, x0)
+    t0 = time.perf_counter()
+    record(core, steps, every)
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    backward(core, [Wu, Wv, None, None])
+    t_bwd = time.perf_counter() - t0
+    core.close()
+    return {"workload": "dry vortex in shear 192x192x40, dx=20 km, dz=500 m, dt=60 s, FP64", "steps": steps,
+            "checkpoint_every": every, "forward_s": round(t_fwd, 4), "record_s": round(t_rec, 4), "backward_s": round(t_bwd, 4),
+            "backward_over_forward": round(t_bwd / t_fwd, 3), "record_plus_backward_over_forward": round((t_rec + t_bwd) / t_fwd, 3)}
 
 
 def cpu_baseline(case, budget_s=20.0):
@@ -113,6 +207,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would shard); "
@@ -160,6 +255,15 @@ def main():
         "adjoint_forward_ratio": None,
         "roofline": roof,
     }
+    if not args.no_extras:
+        log("bench: ensemble (32 members)")
+        out["ensemble"] = ensemble_bench(max(10, args.steps // 5), 3)
+        log(f"bench: ensemble {out['ensemble']['value']:.4g} gridpoint-steps/s, "
+            f"{out['ensemble']['batched_over_one_by_one']}x the members one by one")
+        log("bench: adjoint (192x192x40, 360 steps)")
+        out["adjoint"] = adjoint_bench()
+        out["adjoint_forward_ratio"] = out["adjoint"]["backward_over_forward"]
+        log(f"bench: adjoint backward/forward {out['adjoint_forward_ratio']}")
     if not args.no_cpu:
         log("bench: timing the CPU restatement (oracle) on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(case)

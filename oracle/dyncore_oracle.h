@@ -8,6 +8,26 @@
  *
  * Field, role and column identifiers match include/dyncore.h so that a test can
  * move arrays between the two without a translation table.
+ *
+ * What pins each part (the reference ships no tests, golden vectors or fixtures, and its
+ * serial build needs fftw3.h, netcdf.h and mpi.h, absent here, so it is not compiled):
+ *   orc_fft1d / orc_fft2d      numpy.fft on random data, every transform length the solver
+ *                              uses (tests/test_oracle_math.py)
+ *   orc_interp5/3/2            exactness for cell averages of polynomials up to the stencil's
+ *                              order and the closed form of the upwind term, both signs
+ *   orc_poisson_rhs            manufactured solution: the discrete operator applied to a random
+ *                              periodic field is inverted to 1e-12, with the (0,0) gauge
+ *   orc_kessler_point          per-point conservation of qv+qc+qr, the latent-heat balance,
+ *                              saturation after adjustment, inactive-branch identities
+ *   orc_fall_speed             closed form (kessler.cpp:222-252)
+ *   orc_advect_* , orc_pgf,    closed-form single-term cases: uniform fields, linear profiles,
+ *   orc_solve_pressure,        staggered advecting velocities, buoyancy overwrite, ground flux,
+ *   orc_apply_bc, orc_rayleigh projection to zero divergence, sponge/periodic/lid ranges, damping
+ *                              profile, rest state (tests/test_operators_closed_form.py)
+ *   orc_step                   the composition: parity unpinned against the reference's own
+ *                              serial build (not buildable here); its operator order, written
+ *                              ranges and buffer rotation follow driver_serial.cpp:178-286 and
+ *                              solver.cpp:229-263 line by line.
  */
 #ifndef DYNCORE_ORACLE_H
 #define DYNCORE_ORACLE_H
