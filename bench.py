@@ -17,8 +17,8 @@ profiles/ when one exists for that kernel.
 
 Two more measurements ride along (skip them with --no-extras):
   ensemble  32 dry FP32 vortex members, horizontal sizes drawn from {96,128,160,192}^2, nz=40, advanced
-            by one persistent launch per pass from a hipGraph (launch shape autotuned), beside the same
-            members stepped one after another;
+            by one launch per pass over all members from a hipGraph (launch shape autotuned), beside the
+            same members stepped as separate contexts (each its own stream and graph, so they overlap);
   adjoint   the hand-written adjoint of the dry FP64 step on the 192x192x40 vortex-in-shear case, 360
             steps with a snapshot every 20: backward time / forward time.
 """
@@ -133,7 +133,7 @@ def ensemble_bench(steps, warmup):
     ens.step(steps)
     ens.sync()
     t_ens = time.perf_counter() - t0
-    for m in members:  # each member alone, its own graph, one after another
+    for m in members:  # each member alone: its own stream and graph, launched back to back (they overlap)
         m.step(warmup)
     for m in members:
         m.sync()

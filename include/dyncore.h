@@ -152,7 +152,7 @@ int dyn_adjoint_release(dyn_ctx *ctx);
 /* ---- ensembles ----------------------------------------------------------------------------- */
 /* No counterpart in the reference (its ensemble driver runs members one after another on the CPU).
  * An ensemble advances existing contexts of possibly different NX, NY (same precision and NZ, dry,
- * NZ <= 64) together: every pass of the step is one persistent launch over all members, and
+ * NZ <= 64) together: every pass of the step is one launch over all members' work, and
  * dyn_ensemble_step replays a captured hipGraph.  Members must outlive the ensemble; call
  * dyn_ensemble_synchronize before reading or writing a member's fields.  Per member the results are
  * bit for bit those of dyn_step on that member alone. */

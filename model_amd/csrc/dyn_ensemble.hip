@@ -1,7 +1,7 @@
 // Ensembles: members of uneven grid sizes advanced together.  Every pass of the step (stage advection,
 // the three pressure passes, the boundary epilogue, advance_outer's copy) is one launch over all members:
-// a persistent grid whose blocks walk the members' concatenated work items through a device-resident
-// member table.  A step is captured once into a hipGraph (two steps: the buffer rotation has period 2).
+// the members' work items are concatenated and each block finds its member in a device-resident member
+// table.  A step is captured once into a hipGraph (two steps: the buffer rotation has period 2).
 //
 // All members must share the precision and NZ and be dry (microphysics 0).  The members stay ordinary
 // contexts: their field accessors work between ensemble steps after dyn_ensemble_synchronize.

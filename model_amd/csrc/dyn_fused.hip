@@ -88,7 +88,7 @@ template <typename T, int TJ, bool MOIST>
 __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, const StageArgs &a, T step, int t, unsigned char *smem_raw) {
     using L = MomLds<T, TJ>;
     constexpr int R = L::R, RW = L::RW;
-    constexpr int HU = (R + TJ - 1) / TJ, HW = (RW + TJ - 1) / TJ;  // plane rows per wavefront
+    constexpr int HH = (6 + TJ - 1) / TJ, HW = (RW + TJ - 1) / TJ;  // halo / W plane rows per wavefront
     T *uvp = (T *)smem_raw;
     T *wpl = uvp + L::uv;
     T *nface = wpl + L::w;
@@ -109,14 +109,19 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
     auto UV = [&](int slot, int f, int row) -> T * { return uvp + ((slot * 4 + f) * R + row) * KP; };
     auto WW = [&](int slot, int f, int row) -> T * { return wpl + ((slot * 2 + f) * RW + row) * KP; };
     const void *qsrc[4] = {a.cur[0], a.bar[0], a.cur[1], a.bar[1]};
+    T q[4][6], qs[4];  // U UB V VB of the own row at columns i-2 .. i+3 (i-3 .. i+2 in the prologue)
 
-    // plane rows: U UB of plane iu, V VB W WB of plane iv
-    T su[HU][4], sw[HW][2];
+    // plane rows: U UB of plane iu, V VB W WB of plane iv.  The own row of U UB V VB comes from the
+    // queues (which hold those columns already); only the six halo rows j0-3..j0-1, j0+TJ..j0+TJ+2 are
+    // loaded
+    auto halo_row = [&](int hh) { return hh < 3 ? hh : TJ + hh; };
+    T su[HH][4], sw[HW][2];
     auto fetch_planes = [&](int iu, int iv) {
 #pragma unroll
-        for (int h = 0; h < HU; h++) {
-            const int rr = r + h * TJ;
-            if (rr < R) {
+        for (int h = 0; h < HH; h++) {
+            const int hh = r + h * TJ;
+            if (hh < 6) {
+                const int rr = halo_row(hh);
                 const int jj = min(max(j0 - 3 + rr, 0), NY - 1);
                 const unsigned pu = (unsigned)iu * NYNZ + (unsigned)(jj * NZ + kc), pv = (unsigned)iv * NYNZ + (unsigned)(jj * NZ + kc);
                 su[h][0] = ld<T>(a.cur[0], pu);
@@ -136,12 +141,18 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
             }
         }
     };
-    auto store_planes = [&](int iu, int iv) {
+    // qu, qv: queue positions of columns iu and iv
+    auto store_planes = [&](int iu, int iv, int qu, int qv) {
         const int su_slot = iu % 3, sv_slot = iv % 3;
+        UV(su_slot, 0, RO)[k] = q[0][qu];
+        UV(su_slot, 1, RO)[k] = q[1][qu];
+        UV(sv_slot, 2, RO)[k] = q[2][qv];
+        UV(sv_slot, 3, RO)[k] = q[3][qv];
 #pragma unroll
-        for (int h = 0; h < HU; h++) {
-            const int rr = r + h * TJ;
-            if (rr < R) {
+        for (int h = 0; h < HH; h++) {
+            const int hh = r + h * TJ;
+            if (hh < 6) {
+                const int rr = halo_row(hh);
                 UV(su_slot, 0, rr)[k] = su[h][0];
                 UV(su_slot, 1, rr)[k] = su[h][1];
                 UV(sv_slot, 2, rr)[k] = su[h][2];
@@ -170,15 +181,14 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
     };
 
     // prologue: queues at columns i0-3 .. i0+2, planes U(i0, i0+1), V/W(i0-1, i0)
-    T q[4][6], qs[4];
 #pragma unroll
     for (int f = 0; f < 4; f++)
 #pragma unroll
         for (int d = 0; d < 6; d++) q[f][d] = ld<T>(qsrc[f], (unsigned)(i0 - 3 + d) * NYNZ + rowoff);
     fetch_planes(i0, i0 - 1);
-    store_planes(i0, i0 - 1);
+    store_planes(i0, i0 - 1, 3, 2);
     fetch_planes(i0 + 1, i0);
-    store_planes(i0 + 1, i0);
+    store_planes(i0 + 1, i0, 4, 3);
     fetch_col(i0, cb);
 
     const T dtx = T(g.dtx), dty = T(g.dty), dt = T(g.dt);
@@ -318,7 +328,7 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
         west[0] = east[0];
         west[1] = east[1];
         if (more) {
-            store_planes(i + 2, i + 1);
+            store_planes(i + 2, i + 1, 4, 3);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
 #pragma unroll
@@ -350,7 +360,7 @@ template <typename T, int TJ, bool MOIST>
 __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, const StageArgs &a, T step, int t, unsigned char *smem_raw) {
     using L = ScalLds<T, TJ, MOIST>;
     constexpr int R = L::R, NS = L::NS, NFS = L::NFS;
-    constexpr int HS = (R + TJ - 1) / TJ;
+    constexpr int HH = (6 + TJ - 1) / TJ;  // halo plane rows per wavefront
     T *spl = (T *)smem_raw;
     T *nface = spl + L::s;
 
@@ -370,26 +380,33 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
 
     auto SP = [&](int slot, int f) -> T * { return spl + (slot * NS + f) * R * KP; };
     const void *src[6] = {a.cur[3], a.bar[3], a.cur[4], a.bar[4], a.cur[5], a.cur[6]};
+    T q[NS][6], qs[NS];  // the own row at columns i-2 .. i+3 (i-3 .. i+2 in the prologue)
 
-    T ss[HS][NS];
+    // plane rows: the own row comes from the queue, the six halo rows j0-3..j0-1, j0+TJ..j0+TJ+2 are loaded
+    auto halo_row = [&](int hh) { return hh < 3 ? hh : TJ + hh; };
+    T ss[HH][NS];
     auto fetch_plane = [&](int i) {
 #pragma unroll
-        for (int h = 0; h < HS; h++) {
-            const int rr = r + h * TJ;
-            if (rr < R) {
-                const int jj = min(max(j0 - 3 + rr, 0), NY - 1);
+        for (int h = 0; h < HH; h++) {
+            const int hh = r + h * TJ;
+            if (hh < 6) {
+                const int jj = min(max(j0 - 3 + halo_row(hh), 0), NY - 1);
                 const unsigned pp = (unsigned)i * NYNZ + (unsigned)(jj * NZ + kc);
 #pragma unroll
                 for (int f = 0; f < NS; f++) ss[h][f] = ld<T>(src[f], pp);
             }
         }
     };
-    auto store_plane = [&](int i) {
+    // qd: queue position of column i
+    auto store_plane = [&](int i, int qd) {
         const int slot = i % 2;
 #pragma unroll
-        for (int h = 0; h < HS; h++) {
-            const int rr = r + h * TJ;
-            if (rr < R) {
+        for (int f = 0; f < NS; f++) SP(slot, f)[RO * KP + k] = q[f][qd];
+#pragma unroll
+        for (int h = 0; h < HH; h++) {
+            const int hh = r + h * TJ;
+            if (hh < 6) {
+                const int rr = halo_row(hh);
 #pragma unroll
                 for (int f = 0; f < NS; f++) SP(slot, f)[rr * KP + k] = ss[h][f];
             }
@@ -414,7 +431,6 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         if (MOIST) o.vtt = ld<T>(a.vt, (unsigned)i * NYNZ + topoff);
     };
 
-    T q[NS][6], qs[NS];
 #pragma unroll
     for (int f = 0; f < NS; f++)
 #pragma unroll
@@ -453,7 +469,7 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
     Col cc{}, cs{};
     fetch_col(i0, cc);
     fetch_plane(i0);
-    store_plane(i0);
+    store_plane(i0, 2);
     __syncthreads();
 
     for (int i = i0; i < i1; i++) {
@@ -600,7 +616,7 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
 #pragma unroll
         for (int f = 0; f < NFS; f++) west[f] = east[f];
         if (more) {
-            store_plane(i + 1);
+            store_plane(i + 1, 3);
 #pragma unroll
             for (int f = 0; f < NS; f++) {
 #pragma unroll
@@ -836,7 +852,7 @@ __global__ void __launch_bounds__(256) k_post_scalars(Geo g, Cols c, PostArgs a,
     post_scalars_col<T, MOIST>(g, c, a, clip, blockIdx.x, blockIdx.y + 3);
 }
 
-// ---- batched over the members of an ensemble (persistent blocks walking the concatenated work) --------
+// ---- batched over the members of an ensemble: one block per work item of the concatenated work list ----
 __device__ __forceinline__ StageArgs ens_stage_args(const EnsMember &e, const EnsRoles &ro, int ti, int nbj) {
     StageArgs a;
     for (int f = 0; f < 7; f++) {
@@ -873,40 +889,36 @@ __device__ __forceinline__ PostArgs ens_post_args(const EnsMember &e, const EnsR
 template <typename T, int TJ, bool MOIST>
 __global__ void __launch_bounds__(KP *TJ) k_ens_momentum(const EnsMember *mem, int nmem, EnsRoles ro, T step, int ti, int total) {
     extern __shared__ unsigned char smem_raw[];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::mom0)];
-        const StageArgs a = ens_stage_args(e, ro, ti, e.nbj_mom);
-        __syncthreads();  // the previous tile's LDS reads are done
-        adv_momentum_tile<T, TJ, MOIST>(e.g, e.c, a, step, item - e.mom0, smem_raw);
-    }
+    const int item = blockIdx.x;
+    const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::mom0)];
+    const StageArgs a = ens_stage_args(e, ro, ti, e.nbj_mom);
+    adv_momentum_tile<T, TJ, MOIST>(e.g, e.c, a, step, item - e.mom0, smem_raw);
 }
 
 template <typename T, int TJ, bool MOIST>
 __global__ void __launch_bounds__(KP *TJ) k_ens_scalars(const EnsMember *mem, int nmem, EnsRoles ro, T step, int ti, int total) {
     extern __shared__ unsigned char smem_raw[];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::scal0)];
-        const StageArgs a = ens_stage_args(e, ro, ti, e.nbj_scal);
-        __syncthreads();
-        adv_scalars_tile<T, TJ, MOIST>(e.g, e.c, a, step, item - e.scal0, smem_raw);
-    }
+    const int item = blockIdx.x;
+    const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::scal0)];
+    const StageArgs a = ens_stage_args(e, ro, ti, e.nbj_scal);
+    adv_scalars_tile<T, TJ, MOIST>(e.g, e.c, a, step, item - e.scal0, smem_raw);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_ens_post(const EnsMember *mem, int nmem, EnsRoles ro, T step, bool pgf, bool scalars, int total) {
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::post0)];
-        const int local = item - e.post0, nb = (e.g.NY + 3) / 4;
-        const PostArgs a = ens_post_args(e, ro);
-        post_uvw_col<T>(e.g, e.c, a, step, pgf, local % nb, local / nb + 3);
-        if (scalars) post_scalars_col<T, false>(e.g, e.c, a, false, local % nb, local / nb + 3);
-    }
+    const int item = blockIdx.x;
+    const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::post0)];
+    const PostArgs a = ens_post_args(e, ro);
+    const int local = item - e.post0, nb = (e.g.NY + 3) / 4;
+    post_uvw_col<T>(e.g, e.c, a, step, pgf, local % nb, local / nb + 3);
+    if (scalars) post_scalars_col<T, false>(e.g, e.c, a, false, local % nb, local / nb + 3);
 }
 
 // advance_outer's memcpy(M <- P) for u, v, w, theta of every member; work items are 1024-element chunks
 template <typename T>
 __global__ void __launch_bounds__(256) k_ens_copy(const EnsMember *mem, int nmem, EnsRoles ro, int total) {
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    {
+        const int item = blockIdx.x;
         const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::cp0)];
         const long long n = (long long)e.g.NX * e.g.NYNZ;
         const long long per = (n + 1023) / 1024;
@@ -1030,27 +1042,11 @@ int launch_zero_moisture_shell(Ctx *c, int il, int ih, int jl, int jh) {
 
 
 // ---- ensemble launchers ----------------------------------------------------------------------------
-int cu_count() {
-    static int n = 0;
-    if (!n) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) n = 256;
-    }
-    return n;
-}
-// persistent grid: as many blocks as can be resident at once, never more than there is work
-int persistent_grid(const void *kernel, int threads, size_t lds, int total) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1) per_cu = 1;
-    return std::max(1, std::min(total, per_cu * cu_count()));
-}
-
 namespace {
 template <typename T, int TJ>
 int ens_momentum_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, T step, int ti, int total) {
     const size_t sh = MomLds<T, TJ>::bytes;
-    const void *k = (const void *)k_ens_momentum<T, TJ, false>;
-    const int grid = persistent_grid(k, KP * TJ, sh, total);
+    const int grid = total;
     Timed _t(c0, "ens_momentum", 0.0);
     hipLaunchKernelGGL((k_ens_momentum<T, TJ, false>), dim3(grid), dim3(KP, TJ), sh, c0->stream, d, nmem, ro, step, ti, total);
     DYN_LAUNCH_CHECK();
@@ -1059,8 +1055,7 @@ int ens_momentum_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, T 
 template <typename T, int TJ>
 int ens_scalars_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, T step, int ti, int total) {
     const size_t sh = ScalLds<T, TJ, false>::bytes;
-    const void *k = (const void *)k_ens_scalars<T, TJ, false>;
-    const int grid = persistent_grid(k, KP * TJ, sh, total);
+    const int grid = total;
     Timed _t(c0, "ens_scalars", 0.0);
     hipLaunchKernelGGL((k_ens_scalars<T, TJ, false>), dim3(grid), dim3(KP, TJ), sh, c0->stream, d, nmem, ro, step, ti, total);
     DYN_LAUNCH_CHECK();
@@ -1079,7 +1074,7 @@ int ens_stage_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, doubl
 }
 template <typename T>
 int ens_post_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, bool pgf, int total) {
-    const int grid = persistent_grid((const void *)k_ens_post<T>, 256, 0, total);
+    const int grid = total;
     Timed _t(c0, "ens_post", 0.0);
     hipLaunchKernelGGL(k_ens_post<T>, dim3(grid), dim3(KP, 4), 0, c0->stream, d, nmem, ro, T(step), pgf, true, total);
     DYN_LAUNCH_CHECK();
@@ -1087,7 +1082,7 @@ int ens_post_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double
 }
 template <typename T>
 int ens_copy_t(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, int total) {
-    const int grid = persistent_grid((const void *)k_ens_copy<T>, 256, 0, total);
+    const int grid = total;
     Timed _t(c0, "ens_advance_copy", 0.0);
     hipLaunchKernelGGL(k_ens_copy<T>, dim3(grid), dim3(256), 0, c0->stream, d, nmem, ro, total);
     DYN_LAUNCH_CHECK();

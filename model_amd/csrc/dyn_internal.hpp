@@ -106,12 +106,16 @@ struct EnsRoles {
     unsigned char r[7][3];  // buffer index of (field, role)
 };
 // member owning work item `item` of a batched launch whose per-member offsets are the field `first`
+// (offsets are non-decreasing; a member without work has the next member's offset)
 __device__ __forceinline__ int ens_member(const EnsMember *mem, int nmem, int item, int EnsMember::*first) {
-    int m = 0;
-    while (m + 1 < nmem && mem[m + 1].*first <= item) m++;
-    return m;
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (mem[mid].*first <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
-
 struct AdjState;  // trajectory and adjoint buffers of the hand-written adjoint (dyn_adjoint.hip)
 
 struct Ctx {
@@ -185,8 +189,6 @@ typedef int (*StageHook)(Ctx *c, int stage, void *arg);
 int step_with_hook(Ctx *c, StageHook hook, void *arg);
 bool fused_supported(const Ctx *c);
 // ensembles: batched launches over a device table of members (dyn_fused.hip, dyn_pressure.hip)
-int cu_count();
-int persistent_grid(const void *kernel, int threads, size_t lds, int total);
 int ens_prepare(bool fp32);
 int launch_ens_stage(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, const int *shape, int tot_mom, int tot_scal);
 int launch_ens_post(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, bool pgf, int total);

@@ -676,11 +676,10 @@ __global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, R
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_ens_div_rfft_x(const EnsMember *mem, int nmem, EnsRoles roles, int total, double step) {
     extern __shared__ unsigned char smem[];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const int m = ens_member(mem, nmem, item, &EnsMember::px0);
-        const EnsMember &e = mem[m];
+    {
+        const int item = blockIdx.x;
+        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::px0)];
         const int local = item - e.px0, nch = (e.nk2 + e.kcx - 1) / e.kcx;
-        __syncthreads();
         div_rfft_x_block<T>(e.g, e.c, (const T *)e.buf[DYN_U][roles.r[DYN_U][DYN_ROLE_P]], (const T *)e.buf[DYN_V][roles.r[DYN_V][DYN_ROLE_P]],
                             (const T *)e.buf[DYN_W][roles.r[DYN_W][DYN_ROLE_P]], (T *)e.spec, e.rx, e.twx, e.nk2, e.kcx, step,
                             (const T *)nullptr, local / nch, local % nch, smem);
@@ -690,11 +689,10 @@ __global__ void __launch_bounds__(kThreads) k_ens_div_rfft_x(const EnsMember *me
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_ens_irfft_x_out(const EnsMember *mem, int nmem, int total) {
     extern __shared__ unsigned char smem[];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const int m = ens_member(mem, nmem, item, &EnsMember::px0);
-        const EnsMember &e = mem[m];
+    {
+        const int item = blockIdx.x;
+        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::px0)];
         const int local = item - e.px0, nch = (e.nk2 + e.kcx - 1) / e.kcx;
-        __syncthreads();
         irfft_x_out_block<T>(e.g, (const T *)e.spec, (T *)e.pi, e.rx, e.twx, e.nk2, e.kcx, local / nch, local % nch, smem);
     }
 }
@@ -702,11 +700,10 @@ __global__ void __launch_bounds__(kThreads) k_ens_irfft_x_out(const EnsMember *m
 template <typename T>
 __global__ void __launch_bounds__(1024) k_ens_ypass(const EnsMember *mem, int nmem, int total) {
     extern __shared__ unsigned char smem[];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const int m = ens_member(mem, nmem, item, &EnsMember::py0);
-        const EnsMember &e = mem[m];
+    {
+        const int item = blockIdx.x;
+        const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::py0)];
         ThomasArgs ta{e.coskx, e.cosly, e.c.rhou, e.c.rhow, e.c.mu, e.c.mw, e.acoef, e.ffac, e.dx, e.dz, e.ni, e.nj, e.g.NZ};
-        __syncthreads();
         ypass_block<T, false>((T *)e.spec, e.nk2, e.ry, e.twy, ta, 1, item - e.py0, smem);
     }
 }
@@ -1145,19 +1142,19 @@ int launch_pressure_ens(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &r
         lds_y_set = lds_y;
     }
     {
-        const int grid = persistent_grid((const void *)k_ens_div_rfft_x<float>, kThreads, lds_x, tot_px);
+        const int grid = tot_px;
         Timed _t(c0, "ens_p1_div_rfft_x", 0.0);
         hipLaunchKernelGGL(k_ens_div_rfft_x<float>, dim3(grid), dim3(kThreads), lds_x, c0->stream, d, nmem, ro, tot_px, step);
         DYN_LAUNCH_CHECK();
     }
     {
-        const int grid = persistent_grid((const void *)k_ens_ypass<float>, 1024, lds_y, tot_py);
+        const int grid = tot_py;
         Timed _t(c0, "ens_p2_ypass", 0.0);
         hipLaunchKernelGGL(k_ens_ypass<float>, dim3(grid), dim3(1024), lds_y, c0->stream, d, nmem, tot_py);
         DYN_LAUNCH_CHECK();
     }
     {
-        const int grid = persistent_grid((const void *)k_ens_irfft_x_out<float>, kThreads, lds_x, tot_px);
+        const int grid = tot_px;
         Timed _t(c0, "ens_p5_irfft_x_out", 0.0);
         hipLaunchKernelGGL(k_ens_irfft_x_out<float>, dim3(grid), dim3(kThreads), lds_x, c0->stream, d, nmem, tot_px);
         DYN_LAUNCH_CHECK();
