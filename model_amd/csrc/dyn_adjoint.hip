@@ -43,30 +43,17 @@ __device__ __forceinline__ T i3(T pm1, T p0, T p1, T p2, T sign) {
 }
 template <typename T>
 __device__ __forceinline__ T i2(T p0, T p1) { return T(0.5) * (p1 + p0); }
-// d i5 / d operand d (0..5 = pm2 .. p3) at fixed sign
-template <typename T>
-__device__ __forceinline__ T dc5(int d, T s) {
-    const T a = T(1.0 / 60.0);
-    switch (d) {
-    case 0: return a + a * s;
-    case 1: return -T(2.0 / 15.0) - T(5.0) * a * s;
-    case 2: return T(37.0 / 60.0) + T(10.0) * a * s;
-    case 3: return T(37.0 / 60.0) - T(10.0) * a * s;
-    case 4: return -T(2.0 / 15.0) + T(5.0) * a * s;
-    default: return a - a * s;
-    }
+// thread -> point (i, j, k) of the whole array, k fastest (every lane busy whatever NZ is)
+__device__ __forceinline__ bool point_of(const Geo &g, int &i, int &j, int &k) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)g.NX * g.NYNZ) return false;
+    i = (int)(q / g.NYNZ);
+    const int r = (int)(q - (long long)i * g.NYNZ);
+    j = r / g.NZ;
+    k = r - j * g.NZ;
+    return true;
 }
-// d i3 / d operand d (0..3 = pm1 .. p2) at fixed sign
-template <typename T>
-__device__ __forceinline__ T dc3(int d, T s) {
-    const T a = T(1.0 / 12.0);
-    switch (d) {
-    case 0: return -a - a * s;
-    case 1: return T(7.0 / 12.0) + T(3.0) * a * s;
-    case 2: return T(7.0 / 12.0) - T(3.0) * a * s;
-    default: return -a + a * s;
-    }
-}
+inline dim3 point_grid(const Geo &g) { return dim3((unsigned)(((long long)g.NX * g.NYNZ + 255) / 256)); }
 
 template <typename T>
 struct AdvAdjArgs {
@@ -77,35 +64,47 @@ struct AdvAdjArgs {
     int il, ih, jl, jh;
 };
 
-// Transpose of advect_uvw_velocity + advect_theta over the box at fixed upwind signs.  With A = step *
-// (adjoint of the forecast) on the written points, a face flux F enters the tendencies of its two cells
-// with opposite signs, so its adjoint is a difference of A; F = a * Phi_bar(sign a) + (abar + a) *
-// Phi'(sign(abar + a)) then returns (abar + a) * dPhi'/d(operand) to the advected field and
-// (Phi_bar + Phi') * da/d(velocity) to the advecting velocity.  One thread gathers every term that
-// reaches its point.
+// Transpose of advect_uvw_velocity + advect_theta over the box at fixed upwind signs, in two passes.
+// With A = step * (adjoint of the forecast) on the written points, a face flux F enters the tendencies of
+// its two cells with opposite signs, so its adjoint fh is a difference of A.  F = a Phi_bar(sign a) +
+// (abar + a) Phi'(sign(abar + a)) returns (abar + a) dPhi'/d(operand) to the advected field and
+// (Phi_bar + Phi') da/d(velocity) to the advecting velocity.  Pass 1 (k_adj_faces) evaluates, once per
+// face, for the nine face kinds (u, v, theta on east, north and top faces):
+//   Aw = fh (abar + a), As = Aw sign(abar + a), G = fh (Phi_bar + Phi');
+// pass 2 (k_adj_gather) gathers every term that reaches a point, the interpolant's derivative being
+// c_d + sign u_d (centred and upwind coefficients of operand d).
+enum { FEU = 0, FEV, FET, FNU, FNV, FNT, FZU, FZV, FZT, NFACE };
+struct AdjFaces {
+    void *f[3 * NFACE];  // [kind][Aw, As, G], each a full field indexed by the face's low cell
+};
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_advect(Geo g, Cols c, AdvAdjArgs<T> a, T step) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    const int NZ = g.NZ;
-    const int il = a.il, ih = a.ih, jl = a.jl, jh = a.jh;
-    if (k >= NZ || j >= g.NY) return;
-    if (i < il - 3 || i >= ih + 3 || j < jl - 3 || j >= jh + 3) return;
+struct AdjCommon {
+    const Geo &g;
+    const Cols &c;
+    const AdvAdjArgs<T> &a;
+    T step;
+    __device__ long long X(int ii, int jj, int kk) const { return IX(ii, jj, kk); }
+    __device__ bool inb(int ii, int jj, int kk) const {
+        return ii >= a.il && ii < a.ih && jj >= a.jl && jj < a.jh && kk >= 1 && kk <= g.NZ - 2;
+    }
+    __device__ T AU(int ii, int jj, int kk) const { return inb(ii, jj, kk) ? step * a.PU[X(ii, jj, kk)] : T(0); }
+    __device__ T AV(int ii, int jj, int kk) const { return inb(ii, jj, kk) ? step * a.PV[X(ii, jj, kk)] : T(0); }
+    __device__ T ATH(int ii, int jj, int kk) const { return inb(ii, jj, kk) ? step * a.PTH[X(ii, jj, kk)] : T(0); }
+    __device__ T AW(int ii, int jj, int kk) const { return inb(ii, jj, kk) && kk >= 2 ? step * a.PW[X(ii, jj, kk)] : T(0); }
+    __device__ T DTZ(int kk) const { return T(g.dtz) * T(c.mu[kk]); }
+    // tend(k) -= DTZ(k) (rhow[k+1] top(k) - rhow[k] top(k-1)) one_d_rhou[k]: adjoint of the top face kf
+    __device__ T zcoef_lo(int kf) const { return -DTZ(kf) * T(c.rhow[kf + 1]) * T(c.one_d_rhou[kf]); }
+    __device__ T zcoef_hi(int kf) const { return DTZ(kf + 1) * T(c.rhow[kf + 1]) * T(c.one_d_rhou[kf + 1]); }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_faces(Geo g, Cols c, AdvAdjArgs<T> a, AdjFaces fa, T step) {
+    int i, j, k;
+    if (!point_of(g, i, j, k)) return;
+    const int NZ = g.NZ, il = a.il, ih = a.ih, jl = a.jl, jh = a.jh;
+    const AdjCommon<T> m{g, c, a, step};
     auto X = [&](int ii, int jj, int kk) -> long long { return IX(ii, jj, kk); };
-    auto inb = [&](int ii, int jj, int kk) { return ii >= il && ii < ih && jj >= jl && jj < jh && kk >= 1 && kk <= NZ - 2; };
-    auto AU = [&](int ii, int jj, int kk) -> T { return inb(ii, jj, kk) ? step * a.PU[X(ii, jj, kk)] : T(0); };
-    auto AV = [&](int ii, int jj, int kk) -> T { return inb(ii, jj, kk) ? step * a.PV[X(ii, jj, kk)] : T(0); };
-    auto ATH = [&](int ii, int jj, int kk) -> T { return inb(ii, jj, kk) ? step * a.PTH[X(ii, jj, kk)] : T(0); };
-    auto AW = [&](int ii, int jj, int kk) -> T { return inb(ii, jj, kk) && kk >= 2 ? step * a.PW[X(ii, jj, kk)] : T(0); };
-    const T dtx = T(g.dtx), dty = T(g.dty), dt = T(g.dt);
-    auto DTZ = [&](int kk) -> T { return T(g.dtz) * T(c.mu[kk]); };
-    // adjoint of the top face kf of a field whose scaled tendency adjoint is A:
-    // tend(k) -= DTZ(k) (rhow[k+1] top(k) - rhow[k] top(k-1)) one_d_rhou[k]
-    auto zface = [&](auto A, int ii, int jj, int kf) -> T {
-        return -DTZ(kf) * T(c.rhow[kf + 1]) * T(c.one_d_rhou[kf]) * A(ii, jj, kf) +
-               DTZ(kf + 1) * T(c.rhow[kf + 1]) * T(c.one_d_rhou[kf + 1]) * A(ii, jj, kf + 1);
-    };
     auto I5x = [&](const T *F, int e, int jj, int kk, T s) -> T {
         return i5(F[X(e - 2, jj, kk)], F[X(e - 1, jj, kk)], F[X(e, jj, kk)], F[X(e + 1, jj, kk)], F[X(e + 2, jj, kk)], F[X(e + 3, jj, kk)], s);
     };
@@ -116,167 +115,169 @@ __global__ void __launch_bounds__(256) k_adj_advect(Geo g, Cols c, AdvAdjArgs<T>
         if (kf == 1 || kf == NZ - 2) return i2(F[X(ii, jj, kf)], F[X(ii, jj, kf + 1)]);
         return i3(F[X(ii, jj, kf - 1)], F[X(ii, jj, kf)], F[X(ii, jj, kf + 1)], F[X(ii, jj, kf + 2)], s);
     };
-    auto dIz = [&](int kf, int kk, T s) -> T {
-        if (kf == 1 || kf == NZ - 2) return (kk == kf || kk == kf + 1) ? T(0.5) : T(0);
-        const int d = kk - kf + 1;
-        return (d >= 0 && d <= 3) ? dc3<T>(d, s) : T(0);
+    const long long p = X(i, j, k);
+    auto put = [&](int kind, T fh, T wt, T gg) {
+        const T aw = fh * wt;
+        ((T *)fa.f[3 * kind])[p] = aw;
+        ((T *)fa.f[3 * kind + 1])[p] = aw * sgn(wt);
+        ((T *)fa.f[3 * kind + 2])[p] = fh * gg;
     };
-    const bool krow = k >= 1 && k <= NZ - 2;
-    const bool jrow = j >= jl && j < jh;
-    const bool icol = i >= il && i < ih;
-    T gu = T(0), gv = T(0), gw = T(0), gth = T(0);
+    const T dtx = T(g.dtx), dty = T(g.dty);
+    const bool krow = k >= 1 && k <= NZ - 2, jrow = j >= jl && j < jh, icol = i >= il && i < ih;
+    if (i >= il - 1 && i <= ih - 1 && jrow && krow) {  // east faces e = i
+        const int e = i;
+        {
+            const T ua = T(0.5) * (a.U[X(e + 1, j, k)] + a.U[X(e, j, k)]), uba = T(0.5) * (a.UB[X(e + 1, j, k)] + a.UB[X(e, j, k)]);
+            put(FEU, dtx * (m.AU(e + 1, j, k) - m.AU(e, j, k)), uba + ua,
+                I5x(a.UB, e, j, k, sgn(ua)) + I5x(a.U, e, j, k, sgn(uba + ua)));
+        }
+        {
+            const T ua = T(0.5) * (a.U[X(e + 1, j, k)] + a.U[X(e + 1, j - 1, k)]);
+            const T uba = T(0.5) * (a.UB[X(e + 1, j, k)] + a.UB[X(e + 1, j - 1, k)]);
+            put(FEV, dtx * (m.AV(e + 1, j, k) - m.AV(e, j, k)), uba + ua,
+                I5x(a.VB, e, j, k, sgn(ua)) + I5x(a.V, e, j, k, sgn(uba + ua)));
+        }
+        {
+            const T U1 = a.U[X(e + 1, j, k)], UB1 = a.UB[X(e + 1, j, k)];
+            put(FET, dtx * (m.ATH(e + 1, j, k) - m.ATH(e, j, k)), UB1 + U1,
+                I5x(a.THB, e, j, k, sgn(U1)) + I5x(a.TH, e, j, k, sgn(UB1 + U1)));
+        }
+    }
+    if (j >= jl - 1 && j <= jh - 1 && icol && krow) {  // north faces f = j
+        const int f = j;
+        {
+            const T va = T(0.5) * (a.V[X(i, f + 1, k)] + a.V[X(i - 1, f + 1, k)]);
+            const T vba = T(0.5) * (a.VB[X(i, f + 1, k)] + a.VB[X(i - 1, f + 1, k)]);
+            put(FNU, dty * (m.AU(i, f + 1, k) - m.AU(i, f, k)), vba + va,
+                I5y(a.UB, i, f, k, sgn(va)) + I5y(a.U, i, f, k, sgn(vba + va)));
+        }
+        {
+            const T va = T(0.5) * (a.V[X(i, f + 1, k)] + a.V[X(i, f, k)]), vba = T(0.5) * (a.VB[X(i, f + 1, k)] + a.VB[X(i, f, k)]);
+            put(FNV, dty * (m.AV(i, f + 1, k) - m.AV(i, f, k)), vba + va,
+                I5y(a.VB, i, f, k, sgn(va)) + I5y(a.V, i, f, k, sgn(vba + va)));
+        }
+        {
+            const T V1 = a.V[X(i, f + 1, k)], VB1 = a.VB[X(i, f + 1, k)];
+            put(FNT, dty * (m.ATH(i, f + 1, k) - m.ATH(i, f, k)), VB1 + V1,
+                I5y(a.THB, i, f, k, sgn(V1)) + I5y(a.TH, i, f, k, sgn(VB1 + V1)));
+        }
+    }
+    if (krow && icol && jrow) {  // top faces kf = k
+        const int kf = k;
+        auto zf = [&](T A0, T A1) { return m.zcoef_lo(kf) * A0 + m.zcoef_hi(kf) * A1; };
+        {
+            const T wa = T(0.5) * (a.W[X(i, j, kf + 1)] + a.W[X(i - 1, j, kf + 1)]);
+            const T wba = T(0.5) * (a.WB[X(i, j, kf + 1)] + a.WB[X(i - 1, j, kf + 1)]);
+            put(FZU, zf(m.AU(i, j, kf), m.AU(i, j, kf + 1)), wba + wa, Iz(a.UB, i, j, kf, sgn(wa)) + Iz(a.U, i, j, kf, sgn(wba + wa)));
+        }
+        {
+            const T wa = T(0.5) * (a.W[X(i, j, kf + 1)] + a.W[X(i, j - 1, kf + 1)]);
+            const T wba = T(0.5) * (a.WB[X(i, j, kf + 1)] + a.WB[X(i, j - 1, kf + 1)]);
+            put(FZV, zf(m.AV(i, j, kf), m.AV(i, j, kf + 1)), wba + wa, Iz(a.VB, i, j, kf, sgn(wa)) + Iz(a.V, i, j, kf, sgn(wba + wa)));
+        }
+        {
+            const T W1 = a.W[X(i, j, kf + 1)], WB1 = a.WB[X(i, j, kf + 1)];
+            put(FZT, zf(m.ATH(i, j, kf), m.ATH(i, j, kf + 1)), WB1 + W1, Iz(a.THB, i, j, kf, sgn(W1)) + Iz(a.TH, i, j, kf, sgn(WB1 + W1)));
+        }
+    }
+}
 
-    // ---- east faces e (between columns e and e+1), e in [il-1, ih-1], as advected-field stencils ----
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_gather(Geo g, Cols c, AdvAdjArgs<T> a, AdjFaces fa, T step) {
+    int i, j, k;
+    if (!point_of(g, i, j, k)) return;
+    const int NZ = g.NZ, il = a.il, ih = a.ih, jl = a.jl, jh = a.jh;
+    if (i < il - 3 || i >= ih + 3 || j < jl - 3 || j >= jh + 3) return;
+    const AdjCommon<T> m{g, c, a, step};
+    auto F = [&](int kind, int q, int ii, int jj, int kk) -> T { return ((const T *)fa.f[3 * kind + q])[IX(ii, jj, kk)]; };
+    // i5 = sum_d (c6[d] + sign * u5[d]) p_d and i3 = sum_d (c4[d] + sign * u3[d]) p_d
+    const T c6[6] = {T(1.0 / 60.0), -T(2.0 / 15.0), T(37.0 / 60.0), T(37.0 / 60.0), -T(2.0 / 15.0), T(1.0 / 60.0)};
+    const T u5[6] = {T(1.0 / 60.0), -T(5.0 / 60.0), T(10.0 / 60.0), -T(10.0 / 60.0), T(5.0 / 60.0), -T(1.0 / 60.0)};
+    const T c4[4] = {-T(1.0 / 12.0), T(7.0 / 12.0), T(7.0 / 12.0), -T(1.0 / 12.0)};
+    const T u3[4] = {-T(1.0 / 12.0), T(3.0 / 12.0), -T(3.0 / 12.0), T(1.0 / 12.0)};
+    const T dt = T(g.dt);
+    const bool krow = k >= 1 && k <= NZ - 2, jrow = j >= jl && j < jh, icol = i >= il && i < ih;
+    T gu = T(0), gv = T(0), gw = T(0), gth = T(0);
+    // east faces: stencils of u, v, theta; u as its own advecting velocity
     if (jrow && krow) {
         for (int e = max(i - 3, il - 1); e <= min(i + 2, ih - 1); e++) {
             const int d = i - e + 2;
-            {  // u_east: ua = (U(e+1) + U(e)) / 2
-                const T fh = dtx * (AU(e + 1, j, k) - AU(e, j, k));
-                const T ua = T(0.5) * (a.U[X(e + 1, j, k)] + a.U[X(e, j, k)]), uba = T(0.5) * (a.UB[X(e + 1, j, k)] + a.UB[X(e, j, k)]);
-                const T wt = uba + ua, s = sgn(wt);
-                gu += fh * wt * dc5<T>(d, s);
-                if (e == i - 1 || e == i) gu += T(0.5) * fh * (I5x(a.UB, e, j, k, sgn(ua)) + I5x(a.U, e, j, k, s));
-            }
-            {  // v_east: ua = (U(e+1, j) + U(e+1, j-1)) / 2
-                const T fh = dtx * (AV(e + 1, j, k) - AV(e, j, k));
-                const T ua = T(0.5) * (a.U[X(e + 1, j, k)] + a.U[X(e + 1, j - 1, k)]);
-                const T uba = T(0.5) * (a.UB[X(e + 1, j, k)] + a.UB[X(e + 1, j - 1, k)]);
-                const T wt = uba + ua;
-                gv += fh * wt * dc5<T>(d, sgn(wt));
-            }
-            {  // s_east: U1 = U(e+1)
-                const T fh = dtx * (ATH(e + 1, j, k) - ATH(e, j, k));
-                const T wt = a.UB[X(e + 1, j, k)] + a.U[X(e + 1, j, k)];
-                gth += fh * wt * dc5<T>(d, sgn(wt));
-            }
+            gu += F(FEU, 0, e, j, k) * c6[d] + F(FEU, 1, e, j, k) * u5[d];
+            gv += F(FEV, 0, e, j, k) * c6[d] + F(FEV, 1, e, j, k) * u5[d];
+            gth += F(FET, 0, e, j, k) * c6[d] + F(FET, 1, e, j, k) * u5[d];
+            if (e == i - 1 || e == i) gu += T(0.5) * F(FEU, 2, e, j, k);
         }
     }
-    // ---- U as the advecting velocity of v (faces (i-1, j), (i-1, j+1)) and of theta (face i-1); Coriolis ----
     if (krow) {
         const int e = i - 1;
-        if (e >= il - 1 && e <= ih - 1) {
-            for (int jp = j; jp <= j + 1; jp++) {
-                if (jp < jl || jp >= jh) continue;
-                const T fh = dtx * (AV(e + 1, jp, k) - AV(e, jp, k));
-                const T ua = T(0.5) * (a.U[X(e + 1, jp, k)] + a.U[X(e + 1, jp - 1, k)]);
-                const T uba = T(0.5) * (a.UB[X(e + 1, jp, k)] + a.UB[X(e + 1, jp - 1, k)]);
-                gu += T(0.5) * fh * (I5x(a.VB, e, jp, k, sgn(ua)) + I5x(a.V, e, jp, k, sgn(uba + ua)));
-            }
-            if (jrow) {
-                const T fh = dtx * (ATH(i, j, k) - ATH(e, j, k));
-                const T U1 = a.U[X(i, j, k)], UB1 = a.UB[X(i, j, k)];
-                gu += fh * (I5x(a.THB, e, j, k, sgn(U1)) + I5x(a.TH, e, j, k, sgn(UB1 + U1)));
-            }
+        if (e >= il - 1 && e <= ih - 1) {  // U(i, j) advects v at (e, j) and (e, j+1) and theta at e
+            if (j >= jl && j < jh) gu += T(0.5) * F(FEV, 2, e, j, k);
+            if (j + 1 >= jl && j + 1 < jh) gu += T(0.5) * F(FEV, 2, e, j + 1, k);
+            if (jrow) gu += F(FET, 2, e, j, k);
         }
-        // v: tend(i', j') -= dt f_j' (U(i',j') + U(i'+1,j') + U(i',j'-1) + U(i'+1,j'-1)) / 4
+        // Coriolis: v tend -= dt f (U(i',j') + U(i'+1,j') + U(i',j'-1) + U(i'+1,j'-1)) / 4, u tend += dt f (V ...) / 4
         gu -= T(0.25) * dt *
-              (T(c.f[j]) * (AV(i, j, k) + AV(i - 1, j, k)) + (j + 1 < g.NY ? T(c.f[j + 1]) * (AV(i, j + 1, k) + AV(i - 1, j + 1, k)) : T(0)));
-        // u: tend(i', j') += dt f_j' (V(i',j') + V(i',j'+1) + V(i'-1,j') + V(i'-1,j'+1)) / 4
+              (T(c.f[j]) * (m.AV(i, j, k) + m.AV(i - 1, j, k)) + (j + 1 < g.NY ? T(c.f[j + 1]) * (m.AV(i, j + 1, k) + m.AV(i - 1, j + 1, k)) : T(0)));
         gv += T(0.25) * dt *
-              (T(c.f[j]) * (AU(i, j, k) + AU(i + 1, j, k)) + (j >= 1 ? T(c.f[j - 1]) * (AU(i, j - 1, k) + AU(i + 1, j - 1, k)) : T(0)));
+              (T(c.f[j]) * (m.AU(i, j, k) + m.AU(i + 1, j, k)) + (j >= 1 ? T(c.f[j - 1]) * (m.AU(i, j - 1, k) + m.AU(i + 1, j - 1, k)) : T(0)));
     }
-    // ---- north faces f (between rows f and f+1), f in [jl-1, jh-1], as advected-field stencils ----
+    // north faces
     if (icol && krow) {
         for (int f = max(j - 3, jl - 1); f <= min(j + 2, jh - 1); f++) {
             const int d = j - f + 2;
-            {  // u_north: va = (V(i, f+1) + V(i-1, f+1)) / 2
-                const T fh = dty * (AU(i, f + 1, k) - AU(i, f, k));
-                const T va = T(0.5) * (a.V[X(i, f + 1, k)] + a.V[X(i - 1, f + 1, k)]);
-                const T vba = T(0.5) * (a.VB[X(i, f + 1, k)] + a.VB[X(i - 1, f + 1, k)]);
-                const T wt = vba + va;
-                gu += fh * wt * dc5<T>(d, sgn(wt));
-            }
-            {  // v_north: va = (V(f+1) + V(f)) / 2
-                const T fh = dty * (AV(i, f + 1, k) - AV(i, f, k));
-                const T va = T(0.5) * (a.V[X(i, f + 1, k)] + a.V[X(i, f, k)]), vba = T(0.5) * (a.VB[X(i, f + 1, k)] + a.VB[X(i, f, k)]);
-                const T wt = vba + va, s = sgn(wt);
-                gv += fh * wt * dc5<T>(d, s);
-                if (f == j - 1 || f == j) gv += T(0.5) * fh * (I5y(a.VB, i, f, k, sgn(va)) + I5y(a.V, i, f, k, s));
-            }
-            {  // s_north: V1 = V(i, f+1)
-                const T fh = dty * (ATH(i, f + 1, k) - ATH(i, f, k));
-                const T wt = a.VB[X(i, f + 1, k)] + a.V[X(i, f + 1, k)];
-                gth += fh * wt * dc5<T>(d, sgn(wt));
-            }
+            gu += F(FNU, 0, i, f, k) * c6[d] + F(FNU, 1, i, f, k) * u5[d];
+            gv += F(FNV, 0, i, f, k) * c6[d] + F(FNV, 1, i, f, k) * u5[d];
+            gth += F(FNT, 0, i, f, k) * c6[d] + F(FNT, 1, i, f, k) * u5[d];
+            if (f == j - 1 || f == j) gv += T(0.5) * F(FNV, 2, i, f, k);
         }
     }
-    // ---- V as the advecting velocity of u (north faces of columns i, i+1 at row j-1) and of theta ----
     if (krow) {
         const int f = j - 1;
-        if (f >= jl - 1 && f <= jh - 1) {
-            for (int ip = i; ip <= i + 1; ip++) {
-                if (ip < il || ip >= ih) continue;
-                const T fh = dty * (AU(ip, f + 1, k) - AU(ip, f, k));
-                const T va = T(0.5) * (a.V[X(ip, f + 1, k)] + a.V[X(ip - 1, f + 1, k)]);
-                const T vba = T(0.5) * (a.VB[X(ip, f + 1, k)] + a.VB[X(ip - 1, f + 1, k)]);
-                gv += T(0.5) * fh * (I5y(a.UB, ip, f, k, sgn(va)) + I5y(a.U, ip, f, k, sgn(vba + va)));
-            }
-            if (icol) {
-                const T fh = dty * (ATH(i, j, k) - ATH(i, f, k));
-                const T V1 = a.V[X(i, j, k)], VB1 = a.VB[X(i, j, k)];
-                gv += fh * (I5y(a.THB, i, f, k, sgn(V1)) + I5y(a.TH, i, f, k, sgn(VB1 + V1)));
-            }
+        if (f >= jl - 1 && f <= jh - 1) {  // V(i, j) advects u at (i, f) and (i+1, f) and theta at (i, f)
+            if (i >= il && i < ih) gv += T(0.5) * F(FNU, 2, i, f, k);
+            if (i + 1 >= il && i + 1 < ih) gv += T(0.5) * F(FNU, 2, i + 1, f, k);
+            if (icol) gv += F(FNT, 2, i, f, k);
         }
     }
-    // ---- top faces kf (between levels kf and kf+1), kf in [1, NZ-2], as advected-field stencils ----
+    // top faces
     if (icol && jrow) {
         for (int kf = max(k - 2, 1); kf <= min(k + 1, NZ - 2); kf++) {
-            {  // u_top: wa = (W(i, kf+1) + W(i-1, kf+1)) / 2
-                const T fh = zface(AU, i, j, kf);
-                const T wa = T(0.5) * (a.W[X(i, j, kf + 1)] + a.W[X(i - 1, j, kf + 1)]);
-                const T wba = T(0.5) * (a.WB[X(i, j, kf + 1)] + a.WB[X(i - 1, j, kf + 1)]);
-                const T wt = wba + wa;
-                gu += fh * wt * dIz(kf, k, sgn(wt));
-            }
-            {  // v_top: wa = (W(j, kf+1) + W(j-1, kf+1)) / 2
-                const T fh = zface(AV, i, j, kf);
-                const T wa = T(0.5) * (a.W[X(i, j, kf + 1)] + a.W[X(i, j - 1, kf + 1)]);
-                const T wba = T(0.5) * (a.WB[X(i, j, kf + 1)] + a.WB[X(i, j - 1, kf + 1)]);
-                const T wt = wba + wa;
-                gv += fh * wt * dIz(kf, k, sgn(wt));
-            }
-            {  // s_top: W1 = W(kf+1)
-                const T fh = zface(ATH, i, j, kf);
-                const T wt = a.WB[X(i, j, kf + 1)] + a.W[X(i, j, kf + 1)];
-                gth += fh * wt * dIz(kf, k, sgn(wt));
+            if (kf == 1 || kf == NZ - 2) {
+                if (k == kf || k == kf + 1) {
+                    gu += T(0.5) * F(FZU, 0, i, j, kf);
+                    gv += T(0.5) * F(FZV, 0, i, j, kf);
+                    gth += T(0.5) * F(FZT, 0, i, j, kf);
+                }
+            } else {
+                const int d = k - kf + 1;
+                gu += F(FZU, 0, i, j, kf) * c4[d] + F(FZU, 1, i, j, kf) * u3[d];
+                gv += F(FZV, 0, i, j, kf) * c4[d] + F(FZV, 1, i, j, kf) * u3[d];
+                gth += F(FZT, 0, i, j, kf) * c4[d] + F(FZT, 1, i, j, kf) * u3[d];
             }
         }
     }
-    // ---- W as the advecting velocity (face k-1) of u (columns i, i+1), v (rows j, j+1) and theta ----
     {
-        const int kf = k - 1;
+        const int kf = k - 1;  // W(k) advects at the top face of level k-1
         if (kf >= 1 && kf <= NZ - 2) {
             if (jrow) {
-                for (int ip = i; ip <= i + 1; ip++) {
-                    if (ip < il || ip >= ih) continue;
-                    const T fh = zface(AU, ip, j, kf);
-                    const T wa = T(0.5) * (a.W[X(ip, j, k)] + a.W[X(ip - 1, j, k)]), wba = T(0.5) * (a.WB[X(ip, j, k)] + a.WB[X(ip - 1, j, k)]);
-                    gw += T(0.5) * fh * (Iz(a.UB, ip, j, kf, sgn(wa)) + Iz(a.U, ip, j, kf, sgn(wba + wa)));
-                }
+                if (i >= il && i < ih) gw += T(0.5) * F(FZU, 2, i, j, kf);
+                if (i + 1 >= il && i + 1 < ih) gw += T(0.5) * F(FZU, 2, i + 1, j, kf);
             }
             if (icol) {
-                for (int jp = j; jp <= j + 1; jp++) {
-                    if (jp < jl || jp >= jh) continue;
-                    const T fh = zface(AV, i, jp, kf);
-                    const T wa = T(0.5) * (a.W[X(i, jp, k)] + a.W[X(i, jp - 1, k)]), wba = T(0.5) * (a.WB[X(i, jp, k)] + a.WB[X(i, jp - 1, k)]);
-                    gw += T(0.5) * fh * (Iz(a.VB, i, jp, kf, sgn(wa)) + Iz(a.V, i, jp, kf, sgn(wba + wa)));
-                }
+                if (j >= jl && j < jh) gw += T(0.5) * F(FZV, 2, i, j, kf);
+                if (j + 1 >= jl && j + 1 < jh) gw += T(0.5) * F(FZV, 2, i, j + 1, kf);
             }
-            if (icol && jrow) {
-                const T fh = zface(ATH, i, j, kf);
-                const T W1 = a.W[X(i, j, k)], WB1 = a.WB[X(i, j, k)];
-                gw += fh * (Iz(a.THB, i, j, kf, sgn(W1)) + Iz(a.TH, i, j, kf, sgn(WB1 + W1)));
-            }
+            if (icol && jrow) gw += F(FZT, 2, i, j, kf);
         }
         if (icol && jrow && k >= 1) {
             // Z_ADVECT_BASE (advection.cpp:16-18): W(k) enters the base-state term of levels k-1 and k
             const T coef = T(0.5) * T(c.rhow[k]) * (T(c.tb[k]) - T(c.tb[k - 1]));
-            gw -= coef * (DTZ(k - 1) * T(c.one_d_rhou[k - 1]) * ATH(i, j, k - 1) + DTZ(k) * T(c.one_d_rhou[k]) * ATH(i, j, k));
+            gw -= coef * (m.DTZ(k - 1) * T(c.one_d_rhou[k - 1]) * m.ATH(i, j, k - 1) + m.DTZ(k) * T(c.one_d_rhou[k]) * m.ATH(i, j, k));
         }
         // buoyancy (advection.cpp:329-337): WP(k') = WM + step dt g (th(k')/tbv(k') + th(k'-1)/tbv(k'-1)) / 2
-        if (icol && jrow) gth += dt * T(kGrav) * T(0.5) * (AW(i, j, k) + (k + 1 < NZ ? AW(i, j, k + 1) : T(0))) / T(c.tbv[k]);
+        if (icol && jrow) gth += dt * T(kGrav) * T(0.5) * (m.AW(i, j, k) + (k + 1 < NZ ? m.AW(i, j, k + 1) : T(0))) / T(c.tbv[k]);
     }
-    const long long p = X(i, j, k);
+    const long long p = IX(i, j, k);
     a.CU[p] += gu;
     a.CV[p] += gv;
     a.CW[p] += gw;
@@ -309,10 +310,8 @@ __global__ void k_adj_advect_finish(Geo g, T *MU, T *MV, T *MW, T *MTH, T *PU, T
 template <typename T>
 __global__ void k_adj_pgf(Geo g, Cols c, const T *__restrict__ UH, const T *__restrict__ VH, const T *__restrict__ WH,
                           T *__restrict__ PH, T step) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.NZ || j >= g.NY) return;
+    int i, j, k;
+    if (!point_of(g, i, j, k)) return;
     auto inr = [&](int ii, int jj, int kk) { return ii >= 1 && ii < g.NX - 1 && jj >= 1 && jj < g.NY - 1 && kk >= 1 && kk <= g.NZ - 2; };
     const long long p = IX(i, j, k);
     const T sx = step * T(g.dtx), sy = step * T(g.dty);
@@ -441,6 +440,7 @@ struct AdjState {
     std::vector<void *> rec;   // segment step n, stage st, field f at rec[(n * 3 + st) * 4 + f]
     void *adj[kAdjFields][3] = {{nullptr}};
     void *pihat = nullptr, *r1 = nullptr, *r2 = nullptr;
+    void *faces[27] = {nullptr};  // per-face adjoint weights of the advection transpose
     double *stage_dbl = nullptr;  // host-transfer staging (double)
     int rec_base = 0;             // first recorded step of the segment in the record
 };
@@ -458,6 +458,8 @@ void adj_free(Ctx *c) {
             if (p) (void)hipFree(p);
     void *ptrs[] = {s->pihat, s->r1, s->r2, s->stage_dbl};
     for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (void *p : s->faces)
         if (p) (void)hipFree(p);
     delete s;
     c->adj = nullptr;
@@ -522,7 +524,7 @@ int stage_adjoint(Ctx *c, int n, int stage, double step) {
     if (rc) return rc;
     {
         Timed _t(c, "adj_pgf", (double)c->npts);
-        hipLaunchKernelGGL(k_adj_pgf<T>, dim3((g.NZ + 63) / 64, (g.NY + 3) / 4, g.NX), dim3(64, 4), 0, c->stream, g, c->cols,
+        hipLaunchKernelGGL(k_adj_pgf<T>, point_grid(g), dim3(256), 0, c->stream, g, c->cols,
                            (const T *)P(0), (const T *)P(1), (const T *)P(2), (T *)s->pihat, T(step));
         DYN_LAUNCH_CHECK();
     }
@@ -557,8 +559,13 @@ int stage_adjoint(Ctx *c, int n, int stage, double step) {
     a.jl = 3;
     a.jh = NY - 3;
     {
+        AdjFaces fa;
+        static_assert(sizeof(fa.f) == sizeof(s->faces), "face array count");
+        memcpy(fa.f, s->faces, sizeof(fa.f));
         Timed _t(c, "adj_advect", (double)c->npts);
-        hipLaunchKernelGGL(k_adj_advect<T>, dim3((g.NZ + 63) / 64, (g.NY + 3) / 4, g.NX), dim3(64, 4), 0, c->stream, g, c->cols, a, T(step));
+        hipLaunchKernelGGL(k_adj_faces<T>, point_grid(g), dim3(256), 0, c->stream, g, c->cols, a, fa, T(step));
+        DYN_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_adj_gather<T>, point_grid(g), dim3(256), 0, c->stream, g, c->cols, a, fa, T(step));
         DYN_LAUNCH_CHECK();
         auto M = [&](int f) { return (T *)s->adj[f][DYN_ROLE_M]; };
         hipLaunchKernelGGL(k_adj_advect_finish<T>, dim3((g.NZ - 2 + 63) / 64, (a.jh - a.jl + 3) / 4, a.ih - a.il), dim3(64, 4), 0, c->stream, g,
@@ -697,6 +704,7 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
     DYN_HIP(hipMalloc(&s->pihat, s->field_bytes));
     DYN_HIP(hipMalloc(&s->r1, s->field_bytes));
     DYN_HIP(hipMalloc(&s->r2, s->field_bytes));
+    for (auto &p : s->faces) DYN_HIP(hipMalloc(&p, s->field_bytes));
     DYN_HIP(hipMalloc(&s->stage_dbl, sizeof(double) * c->npts));
     // forward pass with a snapshot every `every` steps and one of the final state
     for (int n = 0; n < nsteps; n++) {

@@ -70,6 +70,7 @@ struct PressurePlan {
     double *acoef = nullptr, *ccoef = nullptr; // pressure.cpp:779-782
     void *spec = nullptr;              // spectral work array [kx][ky][kk] complex (T)
     double *ffac = nullptr;            // Thomas multipliers F2 [kx][kk][ky] (FP64)
+    double *zpiv = nullptr;            // Thomas pivot reciprocals [kx][kk][ky] (FP64)
     bool fused_y = false;              // j-transforms and the vertical solve in one LDS-resident kernel
     size_t lds_max = 0;
 };
@@ -94,7 +95,7 @@ struct EnsMember {
     int js, jn;
     // pressure plan
     void *spec;
-    const double *twx, *twy, *ffac, *coskx, *cosly, *acoef;
+    const double *twx, *twy, *ffac, *zpiv, *coskx, *cosly, *acoef;
     Radix rx, ry;
     int ni, nj, nk2, kcx;
     double dx, dz;

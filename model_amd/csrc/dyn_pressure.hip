@@ -159,8 +159,14 @@ __global__ void __launch_bounds__(kThreads) k_div_fft_x(Geo g, Cols c, const T *
     const int kk0 = blockIdx.y * kc;
     const int jj = j == 0 ? 1 : (j == g.NY - 1 ? g.NY - 2 : j);  // mirrored rows, pressure.cpp:1408-1415
     const double sdt = step * g.dt;
-    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
-        int b = idx % kc, i = idx / kc;
+    // element idx = i*kc + b; when kc divides the block size each thread keeps its level b
+    const bool fixed_b = (blockDim.x % kc) == 0;
+    const int di = blockDim.x / kc;
+    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc; idx < n * kc; idx += blockDim.x) {
+        if (!fixed_b) {
+            b = idx % kc;
+            i = idx / kc;
+        }
         int kk = kk0 + b;
         T d = T(0);
         if (kk < nk2) {
@@ -223,13 +229,14 @@ __global__ void __launch_bounds__(kThreads) k_fft_y(T *__restrict__ spec, int nj
 __global__ void k_thomas_factors(int ni, int nj, int nk, const double *__restrict__ coskx, const double *__restrict__ cosly,
                                  const double *__restrict__ rhou, const double *__restrict__ rhow, const double *__restrict__ mu,
                                  const double *__restrict__ mw, const double *__restrict__ acoef, const double *__restrict__ ccoef,
-                                 double dx, double dz, double *__restrict__ ffac) {
+                                 double dx, double dz, double *__restrict__ ffac, double *__restrict__ zpiv) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ni * nj) return;
     int kx = q / nj, ky = q % nj;
     int nk2 = nk - 2;
     // layout [kx][kk][ky]: the columns of one kx are adjacent for every level
     double *F = ffac + (long long)kx * nk2 * nj + ky;
+    double *Z = zpiv + (long long)kx * nk2 * nj + ky;
     const long long fs = nj;
     double ck = coskx[kx], cl = cosly[ky];
     // B2 of pressure.cpp:787-791
@@ -237,12 +244,14 @@ __global__ void k_thomas_factors(int ni, int nj, int nk, const double *__restric
     double z = 1.0 / B;
     double Fp = ccoef[1] * z;
     F[0] = Fp;
+    Z[0] = z;
     for (int k = 2; k < nk - 1; k++) {
         if (k < nk - 2) B = rhou[k] * 2.0 * (ck + cl - 2.0) / (dx * dx) - mu[k] * (mw[k + 1] * rhow[k + 1] + mw[k] * rhow[k]) / (dz * dz);
         else B = rhou[nk - 2] * 2.0 * (ck + cl - 2.0) / (dx * dx) - (mu[nk - 2] * mw[nk - 2] * rhow[nk - 2]) / (dz * dz);
         z = 1.0 / (B - acoef[k] * Fp);
         Fp = ccoef[k] * z;
         F[(k - 1) * fs] = Fp;
+        Z[(k - 1) * fs] = z;
     }
 }
 
@@ -317,8 +326,15 @@ __global__ void __launch_bounds__(kThreads) k_ifft_x_out(Geo g, const T *__restr
     __syncthreads();
     int out = lds_fft(re, im, rad, kc, tw, +1);
     const int shift = g.NX - 6;
-    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
-        int b = idx % kc, i = idx / kc, kk = kk0 + b;
+    const bool fixed_b = (blockDim.x % kc) == 0;
+    const int di = blockDim.x / kc;
+    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc - di; idx < n * kc; idx += blockDim.x) {
+        i += di;
+        if (!fixed_b) {
+            b = idx % kc;
+            i = idx / kc;
+        }
+        const int kk = kk0 + b;
         if (kk >= nk2) continue;
         T v = re[out][idx];
         int ia = i + 3;
@@ -344,28 +360,40 @@ __global__ void __launch_bounds__(kThreads) k_ifft_x_out(Geo g, const T *__restr
 
 // in-place Stockham DFT over LDS: element (n, b) at re[n*sn + b*sb], im[...]; every butterfly's inputs are
 // read into registers before a barrier, then written to their output slots
+// In-place Stockham DFT of nbatch sequences of length n held in LDS (element (jj, b) at jj*sn + b*sb, re
+// and im in separate planes); twiddles come from an LDS copy of the table (twl[2e], twl[2e+1] = cos,
+// -sin of 2 pi e / n).  Work item idx = tid + t*nthreads is butterfly jj = idx / nbatch of batch
+// b = idx % nbatch; that split is done once, and the per-pass index arithmetic uses shifts while the
+// running length Ns is a power of two (every pass of the radix-4/2-first plans before a 3 or 5).
 template <typename T, int MAXB>
-__device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, int sb, const double *tw, int sign) {
+__device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, int sb, const T *twl, int sign) {
     const int n = rad.n;
     const int nthr = blockDim.x;
-    int Ns = 1;
+    int bq[MAXB], jq[MAXB];
+#pragma unroll
+    for (int t = 0; t < MAXB; t++) {
+        const int idx = threadIdx.x + t * nthr;
+        bq[t] = idx % nbatch;
+        jq[t] = idx / nbatch;
+    }
+    int Ns = 1, lg = 0;
     for (int s = 0; s < rad.nr; s++) {
         const int R = rad.r[s];
         const int nbf = n / R;
         const int stride = n / (Ns * R);
-        const int total = nbf * nbatch;
+        const bool pow2 = (Ns & (Ns - 1)) == 0;
         T xr[MAXB][5], xi[MAXB][5];
         int oidx[MAXB];
 #pragma unroll
         for (int t = 0; t < MAXB; t++) {
-            int idx = threadIdx.x + t * nthr;
-            if (idx < total) {
-                int b = idx % nbatch, jj = idx / nbatch;
-                int jm = jj % Ns;
+            const int jj = jq[t], b = bq[t];
+            if (jj < nbf) {
+                const int jm = pow2 ? (jj & (Ns - 1)) : jj % Ns;
+                const int jd = pow2 ? (jj >> lg) : jj / Ns;
 #pragma unroll
                 for (int r = 0; r < 5; r++) {
                     if (r < R) {
-                        int e = (jj + r * nbf) * sn + b * sb;
+                        const int e = (jj + r * nbf) * sn + b * sb;
                         xr[t][r] = re[e];
                         xi[t][r] = im[e];
                     }
@@ -374,21 +402,20 @@ __device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, i
 #pragma unroll
                     for (int r = 1; r < 5; r++) {
                         if (r < R) {
-                            T wr, wi;
-                            twiddle(tw, r * jm * stride, sign, wr, wi);
+                            const int e = r * jm * stride;
+                            const T wr = twl[2 * e], wi = sign < 0 ? twl[2 * e + 1] : -twl[2 * e + 1];
                             cmul(xr[t][r], xi[t][r], wr, wi);
                         }
                     }
                 }
                 butterfly(R, xr[t], xi[t], sign);
-                oidx[t] = ((jj / Ns) * Ns * R + jm) * sn + b * sb;
+                oidx[t] = (jd * Ns * R + jm) * sn + b * sb;
             }
         }
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < MAXB; t++) {
-            int idx = threadIdx.x + t * nthr;
-            if (idx < total) {
+            if (jq[t] < nbf) {
 #pragma unroll
                 for (int r = 0; r < 5; r++) {
                     if (r < R) {
@@ -400,11 +427,23 @@ __device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, i
         }
         __syncthreads();
         Ns *= R;
+        if ((Ns & (Ns - 1)) == 0) lg = 31 - __clz(Ns);
     }
 }
 
+// copy the (cos, -sin) table of length n into LDS in the storage type
+template <typename T>
+__device__ __forceinline__ void load_twiddles(T *twl, const double *tw, int n) {
+    for (int e = threadIdx.x; e < 2 * n; e += blockDim.x) twl[e] = T(tw[e]);
+}
+
 constexpr int kMaxBX = 4;   // butterflies per thread in the x passes (register-resident)
-constexpr int kMaxBY = 4;   // and in the fused y pass
+constexpr int kMaxBY = 4;   // and in the fused y pass (FP32, 1024 threads)
+constexpr int kMaxBY64 = 6; // the fused y pass in FP64 runs 512 threads with up to 6 butterflies each
+template <typename T>
+constexpr int ypass_threads() { return sizeof(T) == 4 ? 1024 : 512; }
+template <typename T>
+constexpr int ypass_maxb() { return sizeof(T) == 4 ? kMaxBY : kMaxBY64; }
 
 // Q1: divergence3d_periodic prologue + forward real DFT along i; block = (j, chunk of kc levels, kc even).
 // The periodic column UP(NX-3) is read as UP(3), the value periodic_uvw_boundaries puts there.
@@ -416,12 +455,19 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
                                                  int nk2, int kc, double step, const T *__restrict__ src, int j, int chunk,
                                                  unsigned char *smem) {
     const int n = rad.n, P = kc / 2, nkx = n / 2 + 1;
-    T *re = (T *)smem, *im = re + n * P;
+    T *re = (T *)smem, *im = re + n * P, *twl = im + n * P;
     const int kk0 = chunk * kc;
+    load_twiddles(twl, tw, n);
     const int jj = j == 0 ? 1 : (j == g.NY - 1 ? g.NY - 2 : j);  // mirrored rows, pressure.cpp:1408-1415
     const double sdt = step * g.dt;
-    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
-        int b = idx % kc, i = idx / kc;
+    // element idx = i*kc + b; when kc divides the block size each thread keeps its level b
+    const bool fixed_b = (blockDim.x % kc) == 0;
+    const int di = blockDim.x / kc;
+    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc; idx < n * kc; idx += blockDim.x) {
+        if (!fixed_b) {
+            b = idx % kc;
+            i = idx / kc;
+        }
         int kk = kk0 + b;
         T d = T(0);
         if (kk < nk2 && src) {
@@ -437,9 +483,10 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
         }
         if (b & 1) im[i * P + (b >> 1)] = d;
         else re[i * P + (b >> 1)] = d;
+        i += di;
     }
     __syncthreads();
-    lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, tw, -1);
+    lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, twl, -1);
     // split the packed pair: A = (Z[m] + conj Z[-m]) / 2, B = (Z[m] - conj Z[-m]) / 2i
     for (int idx = threadIdx.x; idx < nkx * P; idx += blockDim.x) {
         int pp = idx % P, m = idx / P;
@@ -463,8 +510,9 @@ template <typename T>
 __device__ __forceinline__ void irfft_x_out_block(const Geo &g, const T *__restrict__ spec, T *__restrict__ PI, const Radix &rad,
                                                   const double *tw, int nk2, int kc, int j, int chunk, unsigned char *smem) {
     const int n = rad.n, P = kc / 2, half = n / 2;
-    T *re = (T *)smem, *im = re + n * P;
+    T *re = (T *)smem, *im = re + n * P, *twl = im + n * P;
     const int kk0 = chunk * kc;
+    load_twiddles(twl, tw, n);
     // rebuild Z = A + iB for m in [0, n) from the half spectra of the two levels of each pair
     for (int idx = threadIdx.x; idx < n * P; idx += blockDim.x) {
         int pp = idx % P, m = idx / P;
@@ -492,10 +540,17 @@ __device__ __forceinline__ void irfft_x_out_block(const Geo &g, const T *__restr
         im[m * P + pp] = ai + br;
     }
     __syncthreads();
-    lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, tw, +1);
+    lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, twl, +1);
     const int shift = g.NX - 6;
-    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
-        int b = idx % kc, i = idx / kc, kk = kk0 + b;
+    const bool fixed_b = (blockDim.x % kc) == 0;
+    const int di = blockDim.x / kc;
+    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc - di; idx < n * kc; idx += blockDim.x) {
+        i += di;
+        if (!fixed_b) {
+            b = idx % kc;
+            i = idx / kc;
+        }
+        const int kk = kk0 + b;
         if (kk >= nk2) continue;
         T v = (b & 1) ? im[i * P + (b >> 1)] : re[i * P + (b >> 1)];
         int ia = i + 3, k = kk + 1;
@@ -523,98 +578,72 @@ struct ThomasArgs {
     const double *coskx, *cosly, *rhou, *rhow, *mu, *mw, *acoef, *ffac;
     double dx, dz;
     int ni, nj, nk;
+    const double *zpiv;  // pivot reciprocals z_k of every column, same layout as ffac
 };
 
-// Forward elimination and back substitution of one (kx,ky) column whose values sit at x[kk*xs] (re) and
-// x[kk*xs + is] (im); FP64 arithmetic with the multipliers from the table; the (0,0) column keeps its
-// eliminated values in the FP64 scratch `mean` instead of the storage type (pressure.cpp:837-871).
+// Forward elimination and back substitution of one real sequence (the real or the imaginary part) of one
+// (kx,ky) column whose values sit at x[kk*xs]; FP64 arithmetic with the pivots and multipliers of the
+// tables; the (0,0) column keeps its eliminated values in the FP64 scratch mean[2*kk] instead of the
+// storage type (pressure.cpp:837-871).
 template <typename T>
-__device__ void thomas_column(const ThomasArgs &ta, int kx, int ky, T *x, int xs, int is, double *mean) {
+__device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, double *mean) {
     const int nk = ta.nk, nk2 = nk - 2;
-    const long long fs = ta.nj;
-    const double *F = ta.ffac + (long long)kx * nk2 * ta.nj + ky;
+    const long long fs = ta.nj, col = (long long)kx * nk2 * ta.nj + ky;
+    const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
-    const double ck = ta.coskx[kx], cl = ta.cosly[ky];
     const bool m0 = (kx == 0 && ky == 0);
-    double z = 1.0 / bcoef(1, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz);
-    double pr = (double(x[0]) / size2d) * z, pi = (double(x[is]) / size2d) * z;
-    if (m0) { mean[0] = pr; mean[1] = pi; }
-    else { x[0] = T(pr); x[is] = T(pi); }
-    double Fp = F[0];
+    double p = (double(x[0]) / size2d) * Z[0];
+    if (m0) mean[0] = p;
+    else x[0] = T(p);
+    // unrolled so the table loads of several levels are in flight ahead of the recurrence
+#pragma unroll 8
     for (int k = 2; k < nk - 1; k++) {
-        z = 1.0 / (bcoef(k, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz) - ta.acoef[k] * Fp);
-        int o = (k - 1) * xs;
-        double xr = double(x[o]) / size2d, xi = double(x[o + is]) / size2d;
-        pr = (xr - ta.acoef[k] * pr) * z;
-        pi = (xi - ta.acoef[k] * pi) * z;
-        if (m0) { mean[2 * (k - 1)] = pr; mean[2 * (k - 1) + 1] = pi; }
-        else { x[o] = T(pr); x[o + is] = T(pi); }
-        Fp = F[(k - 1) * fs];
+        const int o = (k - 1) * xs;
+        p = (double(x[o]) / size2d - ta.acoef[k] * p) * Z[(k - 1) * fs];
+        if (m0) mean[2 * (k - 1)] = p;
+        else x[o] = T(p);
     }
-    if (m0) {  // OUT3D(0,0,NZ-2) = 0
-        pr = 0.0;
-        pi = 0.0;
-    }
-    x[(nk2 - 1) * xs] = T(pr);
-    x[(nk2 - 1) * xs + is] = T(pi);
+    if (m0) p = 0.0;  // OUT3D(0,0,NZ-2) = 0
+    x[(nk2 - 1) * xs] = T(p);
+#pragma unroll 8
     for (int k = nk - 3; k >= 1; k--) {
-        int o = (k - 1) * xs;
-        double f = F[(k - 1) * fs];
-        double yr = m0 ? mean[2 * (k - 1)] : double(x[o]);
-        double yi = m0 ? mean[2 * (k - 1) + 1] : double(x[o + is]);
-        pr = yr - f * pr;
-        pi = yi - f * pi;
-        x[o] = T(pr);
-        x[o + is] = T(pi);
+        const int o = (k - 1) * xs;
+        const double y = m0 ? mean[2 * (k - 1)] : double(x[o]);
+        p = y - F[(k - 1) * fs] * p;
+        x[o] = T(p);
     }
 }
 
-// Transpose of thomas_column (the adjoint's vertical solve), in place.  With the pivots z_k =
-// 1/(b_k - a_k F_{k-1}) and the multipliers F_k of the table, the forward map is p = L'^{-1} x / S
+// Transpose of thomas_part (the adjoint's vertical solve), in place.  The forward map is p = L'^{-1} x / S
 // (L' lower bidiagonal: 1/z_k on the diagonal, a_k below) followed by x_k = p_k - F_k x_{k+1}; for the
-// (0,0) column p_{N} is replaced by 0.  The transpose runs the two bidiagonal solves transposed and in
-// reverse order: q_1 = x_1, q_k = x_k - F_{k-1} q_{k-1} (q_N = 0 for (0,0)), then
-// b_N = z_N q_N, b_k = z_k (q_k - a_{k+1} b_{k+1}), result b / S.  The singular last pivot of (0,0)
-// is never formed.
+// (0,0) column p_N is replaced by 0.  The transpose runs the two bidiagonal solves transposed and in
+// reverse order: q_1 = x_1, q_k = x_k - F_{k-1} q_{k-1} (q_N = 0 for (0,0)), then b_N = z_N q_N,
+// b_k = z_k (q_k - a_{k+1} b_{k+1}), result b / S.  The singular last pivot of (0,0) is never used.
 template <typename T>
-__device__ void thomas_column_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs, int is) {
+__device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs) {
     const int nk = ta.nk, N = nk - 2;
-    const long long fs = ta.nj;
-    const double *F = ta.ffac + (long long)kx * N * ta.nj + ky;
+    const long long fs = ta.nj, col = (long long)kx * N * ta.nj + ky;
+    const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
-    const double ck = ta.coskx[kx], cl = ta.cosly[ky];
     const bool m0 = (kx == 0 && ky == 0);
-    double qr = double(x[0]), qi = double(x[is]);
+    double q = double(x[0]);
+#pragma unroll 8
     for (int k = 2; k <= N; k++) {
-        const double f = F[(k - 2) * fs];
         const int o = (k - 1) * xs;
-        qr = double(x[o]) - f * qr;
-        qi = double(x[o + is]) - f * qi;
-        x[o] = T(qr);
-        x[o + is] = T(qi);
+        q = double(x[o]) - F[(k - 2) * fs] * q;
+        x[o] = T(q);
     }
-    auto piv = [&](int k) -> double {
-        if (k == 1) return 1.0 / bcoef(1, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz);
-        return 1.0 / (bcoef(k, nk, ck, cl, ta.rhou, ta.rhow, ta.mu, ta.mw, ta.dx, ta.dz) - ta.acoef[k] * F[(k - 2) * fs]);
-    };
-    double br = 0.0, bi = 0.0;
+    double b = 0.0;
     {
         const int o = (N - 1) * xs;
-        if (!m0) {
-            const double z = piv(N);
-            br = double(x[o]) * z;
-            bi = double(x[o + is]) * z;
-        }
-        x[o] = T(br / size2d);
-        x[o + is] = T(bi / size2d);
+        if (!m0) b = double(x[o]) * Z[(N - 1) * fs];
+        x[o] = T(b / size2d);
     }
+#pragma unroll 8
     for (int k = N - 1; k >= 1; k--) {
         const int o = (k - 1) * xs;
-        const double z = piv(k);
-        br = (double(x[o]) - ta.acoef[k + 1] * br) * z;
-        bi = (double(x[o + is]) - ta.acoef[k + 1] * bi) * z;
-        x[o] = T(br / size2d);
-        x[o + is] = T(bi / size2d);
+        b = (double(x[o]) - ta.acoef[k + 1] * b) * Z[(k - 1) * fs];
+        x[o] = T(b / size2d);
     }
 }
 
@@ -626,6 +655,8 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
     const int nj = rady.n, P2 = nk2 + pad;
     T *re = (T *)smem, *im = re + nj * P2;
     double *mean = (double *)(re + 2 * nj * P2);  // 2*nj*P2 elements of T: 8-byte aligned
+    T *twl = (T *)(mean + 2 * nk2 + 1);
+    load_twiddles(twl, twy, nj);
     const long long base = (long long)kx * nj * nk2 * 2;
     for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
         int kk = idx % nk2, ky = idx / nk2;
@@ -633,15 +664,16 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
         im[ky * P2 + kk] = spec[base + 2 * (long long)idx + 1];
     }
     __syncthreads();
-    lds_fft_ip<T, kMaxBY>(re, im, rady, nk2, P2, 1, twy, -1);
-    if (threadIdx.x < nj) {
-        int ky = threadIdx.x;
-        // re and im of one column sit in separate planes: x[kk] = re[ky*P2 + kk], x[kk + is] = im[...]
-        if (TR) thomas_column_t<T>(ta, kx, ky, re + ky * P2, 1, (int)(im - re));
-        else thomas_column<T>(ta, kx, ky, re + ky * P2, 1, (int)(im - re), mean);
+    lds_fft_ip<T, ypass_maxb<T>()>(re, im, rady, nk2, P2, 1, twl, -1);
+    if ((int)threadIdx.x < 2 * nj) {
+        // one thread per real sequence: threads [0, nj) take the real parts, [nj, 2 nj) the imaginary parts
+        const int part = (int)threadIdx.x >= nj, ky = (int)threadIdx.x - part * nj;
+        T *x = (part ? im : re) + ky * P2;
+        if (TR) thomas_part_t<T>(ta, kx, ky, x, 1);
+        else thomas_part<T>(ta, kx, ky, x, 1, mean + part);
     }
     __syncthreads();
-    lds_fft_ip<T, kMaxBY>(re, im, rady, nk2, P2, 1, twy, +1);
+    lds_fft_ip<T, ypass_maxb<T>()>(re, im, rady, nk2, P2, 1, twl, +1);
     for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
         int kk = idx % nk2, ky = idx / nk2;
         spec[base + 2 * (long long)idx] = re[ky * P2 + kk];
@@ -665,7 +697,8 @@ __global__ void __launch_bounds__(kThreads) k_irfft_x_out(Geo g, const T *__rest
 }
 
 template <typename T, bool TR = false>
-__global__ void __launch_bounds__(1024) k_ypass(T *__restrict__ spec, int nk2, Radix rady, const double *twy, ThomasArgs ta, int pad) {
+__global__ void __launch_bounds__(ypass_threads<T>()) k_ypass(T *__restrict__ spec, int nk2, Radix rady, const double *twy, ThomasArgs ta,
+                                                              int pad) {
     extern __shared__ unsigned char smem[];
     ypass_block<T, TR>(spec, nk2, rady, twy, ta, pad, blockIdx.x, smem);
 }
@@ -703,16 +736,16 @@ __global__ void __launch_bounds__(1024) k_ens_ypass(const EnsMember *mem, int nm
     {
         const int item = blockIdx.x;
         const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::py0)];
-        ThomasArgs ta{e.coskx, e.cosly, e.c.rhou, e.c.rhow, e.c.mu, e.c.mw, e.acoef, e.ffac, e.dx, e.dz, e.ni, e.nj, e.g.NZ};
+        ThomasArgs ta{e.coskx, e.cosly, e.c.rhou, e.c.rhow, e.c.mu, e.c.mw, e.acoef, e.ffac, e.dx, e.dz, e.ni, e.nj, e.g.NZ, e.zpiv};
         ypass_block<T, false>((T *)e.spec, e.nk2, e.ry, e.twy, ta, 1, item - e.py0, smem);
     }
 }
 
 // Q2 (staged): tridiagonal solve of tg columns of one kx staged through LDS (coalesced in and out)
 template <typename T, bool TR = false>
-__global__ void __launch_bounds__(64) k_thomas_staged(T *__restrict__ spec, int nk2, ThomasArgs ta) {
+__global__ void __launch_bounds__(128) k_thomas_staged(T *__restrict__ spec, int nk2, ThomasArgs ta) {
     extern __shared__ unsigned char smem[];
-    const int tg = blockDim.x, P2 = 2 * nk2 + 1;
+    const int tg = blockDim.x / 2, P2 = 2 * nk2 + 1;
     T *x = (T *)smem;
     double *mean = (double *)(smem + ((tg * P2 * sizeof(T) + 7) / 8) * 8);
     const int kx = blockIdx.x, ky0 = blockIdx.y * tg;
@@ -723,9 +756,13 @@ __global__ void __launch_bounds__(64) k_thomas_staged(T *__restrict__ spec, int 
         x[col * P2 + rem] = spec[base + idx];
     }
     __syncthreads();
-    if ((int)threadIdx.x < ncol) {
-        if (TR) thomas_column_t<T>(ta, kx, ky0 + threadIdx.x, x + threadIdx.x * P2, 2, 1);
-        else thomas_column<T>(ta, kx, ky0 + threadIdx.x, x + threadIdx.x * P2, 2, 1, mean);
+    {
+        // two threads per column: even lanes the real part, odd lanes the imaginary part (interleaved)
+        const int colx = (int)threadIdx.x >> 1, part = threadIdx.x & 1;
+        if (colx < ncol) {
+            if (TR) thomas_part_t<T>(ta, kx, ky0 + colx, x + colx * P2 + part, 2);
+            else thomas_part<T>(ta, kx, ky0 + colx, x + colx * P2 + part, 2, mean + part);
+        }
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < ncol * nk2 * 2; idx += blockDim.x) {
@@ -775,6 +812,20 @@ int max_butterflies(const Radix &r, int nbatch, int nthr) {
     return mb;
 }
 
+// LDS of the i passes for kc levels (kc/2 packed pairs) and of the fused j pass, twiddle tables included
+size_t x_pass_lds(int ni, int kc, size_t elt) { return (2 * (size_t)ni * (kc / 2) + 2 * (size_t)ni) * elt; }
+size_t y_pass_lds(int nj, int nk2, size_t elt) {
+    return 2 * (size_t)nj * (nk2 + 1) * elt + 8 * (2 * (size_t)nk2 + 1) + 2 * (size_t)nj * elt + 8;
+}
+// level pairs per block of the i passes: enough blocks to fill the chip, the tile within the LDS budget
+int x_level_chunk(const PressurePlan &pp, const Geo &g, const Radix &rx, size_t elt) {
+    int kc = (int)((kLdsBudget - 2 * (size_t)pp.ni * elt) / (2 * (size_t)pp.ni * elt)) * 2;
+    int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
+    kc = std::max(2, std::min(kc, want));
+    while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
+    return kc;
+}
+
 template <typename T>
 int solve_t(Ctx *c, double step, bool halos) {
     PressurePlan &pp = c->pp;
@@ -790,15 +841,11 @@ int solve_t(Ctx *c, double step, bool halos) {
     T *spec = (T *)pp.spec;
     const double pts = (double)pp.ni * pp.nj * pp.nk2;
     ThomasArgs ta{pp.coskx, pp.cosly, c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz,
-                  pp.ni, pp.nj, c->g.nz};
+                  pp.ni, pp.nj, c->g.nz, pp.zpiv};
     if (pp.ni % 2 == 0) {
         const int nkx = pp.ni / 2 + 1;
-        // level pairs per block: enough blocks to fill the chip, the LDS tile within budget
-        int kc = (int)(kLdsBudget / (2 * (size_t)pp.ni * sizeof(T))) * 2;
-        int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
-        kc = std::max(2, std::min(kc, want));
-        while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
-        size_t shx = 2 * (size_t)pp.ni * (kc / 2) * sizeof(T);
+        const int kc = x_level_chunk(pp, g, rx, sizeof(T));
+        const size_t shx = x_pass_lds(pp.ni, kc, sizeof(T));
         dim3 gx(g.NY, (pp.nk2 + kc - 1) / kc);
         {
             Timed _t(c, "p1_div_rfft_x", pts);
@@ -808,10 +855,10 @@ int solve_t(Ctx *c, double step, bool halos) {
         }
         DYN_LAUNCH_CHECK();
         const int pad = 1;
-        size_t shy = 2 * (size_t)pp.nj * (pp.nk2 + pad) * sizeof(T) + 8 + 16 * (size_t)pp.nk2 + 8;
-        if (pp.fused_y && shy <= pp.lds_max && max_butterflies(ry, pp.nk2, 1024) <= kMaxBY) {
+        const size_t shy = y_pass_lds(pp.nj, pp.nk2, sizeof(T));
+        if (pp.fused_y) {
             Timed _t(c, "p2_ypass", pts);
-            hipLaunchKernelGGL(k_ypass<T>, dim3(nkx), dim3(1024), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
+            hipLaunchKernelGGL(k_ypass<T>, dim3(nkx), dim3(ypass_threads<T>()), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
             DYN_LAUNCH_CHECK();
         } else {
             int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
@@ -826,7 +873,7 @@ int solve_t(Ctx *c, double step, bool halos) {
             size_t sht = ((tg * (2 * (size_t)pp.nk2 + 1) * sizeof(T) + 7) / 8) * 8 + 16 * (size_t)pp.nk2;
             {
                 Timed _t(c, "p3_thomas", pts);
-                hipLaunchKernelGGL(k_thomas_staged<T>, dim3(nkx, (pp.nj + tg - 1) / tg), dim3(tg), sht, c->stream, spec, pp.nk2, ta);
+                hipLaunchKernelGGL(k_thomas_staged<T>, dim3(nkx, (pp.nj + tg - 1) / tg), dim3(2 * tg), sht, c->stream, spec, pp.nk2, ta);
             }
             DYN_LAUNCH_CHECK();
             {
@@ -905,9 +952,9 @@ __global__ void k_adj_pi_fold(Geo g, const T *__restrict__ PH, T *__restrict__ R
 template <typename T>
 __global__ void k_adj_div(Geo g, Cols c, const T *__restrict__ R, T *__restrict__ UH, T *__restrict__ VH, T *__restrict__ WH,
                           double step) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y, x = blockIdx.z;
-    if (k >= g.NZ) return;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)g.NX * g.NYNZ) return;
+    const int x = (int)(q / g.NYNZ), r = (int)(q - (long long)x * g.NYNZ), y = r / g.NZ, k = r - y * g.NZ;
     const T sdt = T(step * g.dt);
     auto Rf = [&](int ia, int yy, int kk) -> T {
         if (ia < 3 || ia >= g.NX - 3 || yy < 1 || yy > g.NY - 2 || kk < 1 || kk > g.NZ - 2) return T(0);
@@ -963,13 +1010,10 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     }
     T *spec = (T *)pp.spec;
     ThomasArgs ta{pp.coskx, pp.cosly, c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz,
-                  pp.ni, pp.nj, c->g.nz};
+                  pp.ni, pp.nj, c->g.nz, pp.zpiv};
     const int nkx = pp.ni / 2 + 1;
-    int kc = (int)(kLdsBudget / (2 * (size_t)pp.ni * sizeof(T))) * 2;
-    int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
-    kc = std::max(2, std::min(kc, want));
-    while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
-    size_t shx = 2 * (size_t)pp.ni * (kc / 2) * sizeof(T);
+    const int kc = x_level_chunk(pp, g, rx, sizeof(T));
+    const size_t shx = x_pass_lds(pp.ni, kc, sizeof(T));
     dim3 gx(g.NY, (pp.nk2 + kc - 1) / kc);
     {
         Timed _t(c, "adj_rfft_x", pts);
@@ -978,10 +1022,10 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     }
     DYN_LAUNCH_CHECK();
     const int pad = 1;
-    size_t shy = 2 * (size_t)pp.nj * (pp.nk2 + pad) * sizeof(T) + 8 + 16 * (size_t)pp.nk2 + 8;
-    if (pp.fused_y && shy <= pp.lds_max && max_butterflies(ry, pp.nk2, 1024) <= kMaxBY) {
+    const size_t shy = y_pass_lds(pp.nj, pp.nk2, sizeof(T));
+    if (pp.fused_y) {
         Timed _t(c, "adj_ypass", pts);
-        hipLaunchKernelGGL((k_ypass<T, true>), dim3(nkx), dim3(1024), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
+        hipLaunchKernelGGL((k_ypass<T, true>), dim3(nkx), dim3(ypass_threads<T>()), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
         DYN_LAUNCH_CHECK();
     } else {
         int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
@@ -992,7 +1036,7 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
         DYN_LAUNCH_CHECK();
         const int tg = 64;
         size_t sht = ((tg * (2 * (size_t)pp.nk2 + 1) * sizeof(T) + 7) / 8) * 8 + 16 * (size_t)pp.nk2;
-        hipLaunchKernelGGL((k_thomas_staged<T, true>), dim3(nkx, (pp.nj + tg - 1) / tg), dim3(tg), sht, c->stream, spec, pp.nk2, ta);
+        hipLaunchKernelGGL((k_thomas_staged<T, true>), dim3(nkx, (pp.nj + tg - 1) / tg), dim3(2 * tg), sht, c->stream, spec, pp.nk2, ta);
         DYN_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
         DYN_LAUNCH_CHECK();
@@ -1004,8 +1048,8 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     DYN_LAUNCH_CHECK();
     {
         Timed _t(c, "adj_div", (double)c->npts);
-        hipLaunchKernelGGL(k_adj_div<T>, dim3((g.NZ + 63) / 64, g.NY, g.NX), dim3(64), 0, c->stream, g, c->cols, (const T *)r2, uh, vh, wh,
-                           step);
+        hipLaunchKernelGGL(k_adj_div<T>, dim3((unsigned)(((long long)g.NX * g.NYNZ + 255) / 256)), dim3(256), 0, c->stream, g, c->cols,
+                           (const T *)r2, uh, vh, wh, step);
     }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
@@ -1032,21 +1076,27 @@ int pressure_init(Ctx *c) {
         return DYN_E_HIP;
     }
     DYN_HIP(hipMalloc(&pp.spec, sizeof(double) * 2 * (size_t)pp.ni * pp.nj * pp.nk2));
-    // the fused j pass keeps a whole (j, k) slab in LDS: FP32 only (FP64 slabs of this size do not fit)
     int dev = 0, lds = 0;
     DYN_HIP(hipGetDevice(&dev));
     DYN_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     pp.lds_max = (size_t)lds;
-    pp.fused_y = c->prec == DYN_FP32;
-    if (pp.fused_y) {
-        size_t need = 2 * (size_t)pp.nj * (pp.nk2 + 1) * sizeof(float) + 8 + 16 * (size_t)pp.nk2 + 8;
-        if (need > pp.lds_max) pp.fused_y = false;
-        else {
-            DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-            DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-        }
+    // the fused j pass keeps a whole (j, k) slab of one kx in LDS when it fits (FP32 at 256x64, FP64 up to
+    // about 192x40) and one thread per real sequence of the solve exists; otherwise the j transforms and
+    // the solve run as three kernels
+    const size_t need = y_pass_lds(pp.nj, pp.nk2, c->elt);
+    const bool f32 = c->prec == DYN_FP32;
+    const int nthr = f32 ? ypass_threads<float>() : ypass_threads<double>();
+    const int maxb = f32 ? ypass_maxb<float>() : ypass_maxb<double>();
+    pp.fused_y = need <= pp.lds_max && 2 * pp.nj <= nthr && max_butterflies(ry, pp.nk2, nthr) <= maxb;
+    if (pp.fused_y && f32) {
+        DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    } else if (pp.fused_y) {
+        DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
     }
     DYN_HIP(hipMalloc(&pp.ffac, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
+    DYN_HIP(hipMalloc(&pp.zpiv, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
     DYN_HIP(hipMalloc(&pp.coskx, sizeof(double) * pp.ni));
     DYN_HIP(hipMalloc(&pp.cosly, sizeof(double) * pp.nj));
     DYN_HIP(hipMalloc(&pp.acoef, sizeof(double) * NZ));
@@ -1078,7 +1128,7 @@ int pressure_coefficients(Ctx *c) {
     DYN_HIP(hipMemcpy(pp.ccoef, cc.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
     int nq = pp.ni * pp.nj;
     hipLaunchKernelGGL(k_thomas_factors, dim3((nq + 127) / 128), dim3(128), 0, c->stream, pp.ni, pp.nj, nk, pp.coskx, pp.cosly,
-                       c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ccoef, c->g.dx, dz, pp.ffac);
+                       c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ccoef, c->g.dx, dz, pp.ffac, pp.zpiv);
     DYN_LAUNCH_CHECK();
     DYN_HIP(hipStreamSynchronize(c->stream));
     return DYN_OK;
@@ -1086,7 +1136,7 @@ int pressure_coefficients(Ctx *c) {
 
 int pressure_free(Ctx *c) {
     PressurePlan &pp = c->pp;
-    void *ptrs[] = {pp.twx, pp.twy, pp.spec, pp.ffac, pp.coskx, pp.cosly, pp.acoef, pp.ccoef};
+    void *ptrs[] = {pp.twx, pp.twy, pp.spec, pp.ffac, pp.zpiv, pp.coskx, pp.cosly, pp.acoef, pp.ccoef};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     pp = PressurePlan{};
@@ -1098,16 +1148,6 @@ int launch_solve_pressure(Ctx *c, double step, bool halos) {
 }
 
 
-namespace {
-template <typename T>
-int level_pairs(const PressurePlan &pp, const Geo &g, const Radix &rx) {
-    int kc = (int)(kLdsBudget / (2 * (size_t)pp.ni * sizeof(T))) * 2;
-    int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
-    kc = std::max(2, std::min(kc, want));
-    while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
-    return kc;
-}
-}  // namespace
 
 // the pressure part of an ensemble member's table entry; false when the member cannot join the batched
 // solve (odd ni, or a (j, k) slab too large for the LDS-resident j pass)
@@ -1115,7 +1155,7 @@ bool pressure_ens_fill(Ctx *c, EnsMember &e, size_t &lds_x, size_t &lds_y) {
     PressurePlan &pp = c->pp;
     if (pp.ni % 2 != 0 || !pp.fused_y || c->prec != DYN_FP32) return false;
     if (!factorize(pp.ni, e.rx) || !factorize(pp.nj, e.ry)) return false;
-    e.kcx = level_pairs<float>(pp, c->geo, e.rx);
+    e.kcx = x_level_chunk(pp, c->geo, e.rx, sizeof(float));
     e.ni = pp.ni;
     e.nj = pp.nj;
     e.nk2 = pp.nk2;
@@ -1123,14 +1163,15 @@ bool pressure_ens_fill(Ctx *c, EnsMember &e, size_t &lds_x, size_t &lds_y) {
     e.twx = pp.twx;
     e.twy = pp.twy;
     e.ffac = pp.ffac;
+    e.zpiv = pp.zpiv;
     e.coskx = pp.coskx;
     e.cosly = pp.cosly;
     e.acoef = pp.acoef;
     e.dx = c->g.dx;
     e.dz = c->g.dz;
-    lds_x = 2 * (size_t)pp.ni * (e.kcx / 2) * sizeof(float);
-    lds_y = 2 * (size_t)pp.nj * (pp.nk2 + 1) * sizeof(float) + 8 + 16 * (size_t)pp.nk2 + 8;
-    return lds_y <= pp.lds_max && max_butterflies(e.ry, pp.nk2, 1024) <= kMaxBY;
+    lds_x = x_pass_lds(pp.ni, e.kcx, sizeof(float));
+    lds_y = y_pass_lds(pp.nj, pp.nk2, sizeof(float));
+    return lds_y <= pp.lds_max && 2 * pp.nj <= 1024 && max_butterflies(e.ry, pp.nk2, 1024) <= kMaxBY;
 }
 
 // solve_pressure(step) of every member without the uvw halo copy (as in the fused step): three launches
