@@ -162,7 +162,8 @@ __global__ void __launch_bounds__(kThreads) k_div_fft_x(Geo g, Cols c, const T *
     // element idx = i*kc + b; when kc divides the block size each thread keeps its level b
     const bool fixed_b = (blockDim.x % kc) == 0;
     const int di = blockDim.x / kc;
-    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc; idx < n * kc; idx += blockDim.x) {
+    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc - di; idx < n * kc; idx += blockDim.x) {
+        i += di;
         if (!fixed_b) {
             b = idx % kc;
             i = idx / kc;
@@ -170,9 +171,11 @@ __global__ void __launch_bounds__(kThreads) k_div_fft_x(Geo g, Cols c, const T *
         int kk = kk0 + b;
         T d = T(0);
         if (kk < nk2) {
+            // the periodic column UP(NX-3) is read as UP(3), the value periodic_uvw_boundaries puts there
             int k = kk + 1, ia = i + 3;
+            int ie = ia + 1 == g.NX - 3 ? 3 : ia + 1;
             T rhou = T(c.rhou[k]);
-            d = rhou * (UP[IX(ia + 1, jj, k)] - UP[IX(ia, jj, k)]) * T(g.one_d_dx) +
+            d = rhou * (UP[IX(ie, jj, k)] - UP[IX(ia, jj, k)]) * T(g.one_d_dx) +
                 rhou * (VP[IX(ia, jj + 1, k)] - VP[IX(ia, jj, k)]) * T(g.one_d_dy) +
                 (T(c.rhow[k + 1]) * WP[IX(ia, jj, k + 1)] - T(c.rhow[k]) * WP[IX(ia, jj, k)]) * (T(g.one_d_dz) * T(c.mu[k]));
             d /= T(sdt);
@@ -463,7 +466,10 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
     // element idx = i*kc + b; when kc divides the block size each thread keeps its level b
     const bool fixed_b = (blockDim.x % kc) == 0;
     const int di = blockDim.x / kc;
-    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc; idx < n * kc; idx += blockDim.x) {
+    // unrolled: the global loads of several elements are in flight before their LDS stores
+#pragma unroll 4
+    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc - di; idx < n * kc; idx += blockDim.x) {
+        i += di;
         if (!fixed_b) {
             b = idx % kc;
             i = idx / kc;
@@ -483,7 +489,6 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
         }
         if (b & 1) im[i * P + (b >> 1)] = d;
         else re[i * P + (b >> 1)] = d;
-        i += di;
     }
     __syncthreads();
     lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, twl, -1);
@@ -514,6 +519,7 @@ __device__ __forceinline__ void irfft_x_out_block(const Geo &g, const T *__restr
     const int kk0 = chunk * kc;
     load_twiddles(twl, tw, n);
     // rebuild Z = A + iB for m in [0, n) from the half spectra of the two levels of each pair
+#pragma unroll 4
     for (int idx = threadIdx.x; idx < n * P; idx += blockDim.x) {
         int pp = idx % P, m = idx / P;
         int ka = kk0 + 2 * pp, kb = ka + 1;
@@ -658,10 +664,23 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
     T *twl = (T *)(mean + 2 * nk2 + 1);
     load_twiddles(twl, twy, nj);
     const long long base = (long long)kx * nj * nk2 * 2;
-    for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
-        int kk = idx % nk2, ky = idx / nk2;
-        re[ky * P2 + kk] = spec[base + 2 * (long long)idx];
-        im[ky * P2 + kk] = spec[base + 2 * (long long)idx + 1];
+    // element idx = ky*nk2 + kk; (kk, ky) advance by the block size without dividing; unrolled so the
+    // global loads of several elements are in flight at once
+    const int dk = (int)blockDim.x % nk2, dy = (int)blockDim.x / nk2;
+    {
+        int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
+#pragma unroll 8
+        for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
+            const T vr = spec[base + 2 * (long long)idx], vi = spec[base + 2 * (long long)idx + 1];
+            re[ky * P2 + kk] = vr;
+            im[ky * P2 + kk] = vi;
+            kk += dk;
+            ky += dy;
+            if (kk >= nk2) {
+                kk -= nk2;
+                ky++;
+            }
+        }
     }
     __syncthreads();
     lds_fft_ip<T, ypass_maxb<T>()>(re, im, rady, nk2, P2, 1, twl, -1);
@@ -674,10 +693,18 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
     }
     __syncthreads();
     lds_fft_ip<T, ypass_maxb<T>()>(re, im, rady, nk2, P2, 1, twl, +1);
-    for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
-        int kk = idx % nk2, ky = idx / nk2;
-        spec[base + 2 * (long long)idx] = re[ky * P2 + kk];
-        spec[base + 2 * (long long)idx + 1] = im[ky * P2 + kk];
+    {
+        int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
+        for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
+            spec[base + 2 * (long long)idx] = re[ky * P2 + kk];
+            spec[base + 2 * (long long)idx + 1] = im[ky * P2 + kk];
+            kk += dk;
+            ky += dy;
+            if (kk >= nk2) {
+                kk -= nk2;
+                ky++;
+            }
+        }
     }
 }
 

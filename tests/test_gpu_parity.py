@@ -230,3 +230,14 @@ def test_fused_launch_shapes_bitwise(precision, shape):
     for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR, ids.VT):
         role = CUR if f in ids.PROGNOSTIC else 0
         assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), ids.FIELD_NAMES[f]
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["per-operator", "graph"])
+def test_odd_transform_length_against_oracle(graph):
+    """ni = 15 and nj = 18 take the full-complex transform path (odd i length) of the pressure solve."""
+    c = configs.small_case(nx=15, ny=18, nz=12, microphysics=0, amplitude=0.3)
+    o, g = _run_pair(c, 6, graph)
+    for f in (ids.U, ids.V, ids.W, ids.TH):
+        e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
+        assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
+    assert rel_err(g.get_field(ids.PI, 0), o.get_field(ids.PI, 0)) <= 1e-11
