@@ -969,9 +969,9 @@ bool fused_supported(const Ctx *c) { return c->g.nz <= KP && (double)c->npts * (
 
 // launch-shape knobs of the stage kernels: rows per block (instantiated widths) and march length
 bool fused_shape_ok(const Ctx *c, int param, int value) {
-    (void)c;
     if (param == DYN_TUNE_MOMENTUM_TJ || param == DYN_TUNE_SCALAR_TJ) return value == 2 || value == 4 || value == 8;
     if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
+    if (param == DYN_TUNE_PRESSURE_JPASS) return value == 2 || (value == 1 && c->pp.fused_y);
     return false;
 }
 
@@ -983,6 +983,7 @@ int fused_shape(const Ctx *c, int param) {
     case DYN_TUNE_SCALAR_TJ: return f32 ? 8 : 4;
     case DYN_TUNE_MOMENTUM_TI:
     case DYN_TUNE_SCALAR_TI: return 16;
+    case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : 2;
     default: return DYN_E_ARG;
     }
 }

@@ -587,36 +587,68 @@ struct ThomasArgs {
     const double *zpiv;  // pivot reciprocals z_k of every column, same layout as ffac
 };
 
+// The sweeps below walk the levels in chunks of kTC and load the next chunk's table values (pivots or
+// multipliers, one column of the [kx][kk][ky] tables) while the current chunk is eliminated, so the
+// dependent recurrence does not wait on global-memory latency level by level.
+constexpr int kTC = 8;
+
 // Forward elimination and back substitution of one real sequence (the real or the imaginary part) of one
 // (kx,ky) column whose values sit at x[kk*xs]; FP64 arithmetic with the pivots and multipliers of the
 // tables; the (0,0) column keeps its eliminated values in the FP64 scratch mean[2*kk] instead of the
-// storage type (pressure.cpp:837-871).
+// storage type (pressure.cpp:837-871).  Level k = kk+1; acoef[1] = 0, so the first level needs no
+// special case.
 template <typename T>
 __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, double *mean) {
-    const int nk = ta.nk, nk2 = nk - 2;
-    const long long fs = ta.nj, col = (long long)kx * nk2 * ta.nj + ky;
+    const int N = ta.nk - 2;
+    const long long fs = ta.nj, col = (long long)kx * N * ta.nj + ky;
     const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
     const bool m0 = (kx == 0 && ky == 0);
-    double p = (double(x[0]) / size2d) * Z[0];
-    if (m0) mean[0] = p;
-    else x[0] = T(p);
-    // unrolled so the table loads of several levels are in flight ahead of the recurrence
-#pragma unroll 8
-    for (int k = 2; k < nk - 1; k++) {
-        const int o = (k - 1) * xs;
-        p = (double(x[o]) / size2d - ta.acoef[k] * p) * Z[(k - 1) * fs];
-        if (m0) mean[2 * (k - 1)] = p;
-        else x[o] = T(p);
+    double cur[kTC], nxt[kTC];
+#pragma unroll
+    for (int t = 0; t < kTC; t++) nxt[t] = t < N ? Z[t * fs] : 0.0;
+    double p = 0.0;
+    for (int k0 = 0; k0 < N; k0 += kTC) {
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            cur[t] = nxt[t];
+            const int kk = k0 + kTC + t;
+            nxt[t] = kk < N ? Z[kk * fs] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            const int kk = k0 + t;
+            if (kk < N) {
+                p = (double(x[kk * xs]) / size2d - ta.acoef[kk + 1] * p) * cur[t];
+                if (m0) mean[2 * kk] = p;
+                else x[kk * xs] = T(p);
+            }
+        }
     }
     if (m0) p = 0.0;  // OUT3D(0,0,NZ-2) = 0
-    x[(nk2 - 1) * xs] = T(p);
-#pragma unroll 8
-    for (int k = nk - 3; k >= 1; k--) {
-        const int o = (k - 1) * xs;
-        const double y = m0 ? mean[2 * (k - 1)] : double(x[o]);
-        p = y - F[(k - 1) * fs] * p;
-        x[o] = T(p);
+    x[(N - 1) * xs] = T(p);
+    // back substitution x_kk = y_kk - F_kk x_{kk+1} for kk = N-2 .. 0, walking down in chunks
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = N - 2 - t;
+        nxt[t] = kk >= 0 ? F[kk * fs] : 0.0;
+    }
+    for (int k0 = N - 2; k0 >= 0; k0 -= kTC) {
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            cur[t] = nxt[t];
+            const int kk = k0 - kTC - t;
+            nxt[t] = kk >= 0 ? F[kk * fs] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            const int kk = k0 - t;
+            if (kk >= 0) {
+                const double y = m0 ? mean[2 * kk] : double(x[kk * xs]);
+                p = y - cur[t] * p;
+                x[kk * xs] = T(p);
+            }
+        }
     }
 }
 
@@ -627,29 +659,55 @@ __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, 
 // b_k = z_k (q_k - a_{k+1} b_{k+1}), result b / S.  The singular last pivot of (0,0) is never used.
 template <typename T>
 __device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs) {
-    const int nk = ta.nk, N = nk - 2;
+    const int N = ta.nk - 2;
     const long long fs = ta.nj, col = (long long)kx * N * ta.nj + ky;
     const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
     const bool m0 = (kx == 0 && ky == 0);
+    double cur[kTC], nxt[kTC];
+    // q_kk = x_kk - F_{kk-1} q_{kk-1}, kk = 1 .. N-1 (q_0 = x_0)
+#pragma unroll
+    for (int t = 0; t < kTC; t++) nxt[t] = t < N - 1 ? F[t * fs] : 0.0;
     double q = double(x[0]);
-#pragma unroll 8
-    for (int k = 2; k <= N; k++) {
-        const int o = (k - 1) * xs;
-        q = double(x[o]) - F[(k - 2) * fs] * q;
-        x[o] = T(q);
+    for (int k0 = 1; k0 < N; k0 += kTC) {
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            cur[t] = nxt[t];
+            const int kk = k0 + kTC + t;
+            nxt[t] = kk < N ? F[(kk - 1) * fs] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            const int kk = k0 + t;
+            if (kk < N) {
+                q = double(x[kk * xs]) - cur[t] * q;
+                x[kk * xs] = T(q);
+            }
+        }
+    }
+    // b_{N-1} = z_{N-1} q_{N-1} (0 for (0,0)); b_kk = z_kk (q_kk - a_{kk+2} b_{kk+1}), kk = N-2 .. 0
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = N - 1 - t;
+        nxt[t] = kk >= 0 ? Z[kk * fs] : 0.0;
     }
     double b = 0.0;
-    {
-        const int o = (N - 1) * xs;
-        if (!m0) b = double(x[o]) * Z[(N - 1) * fs];
-        x[o] = T(b / size2d);
-    }
-#pragma unroll 8
-    for (int k = N - 1; k >= 1; k--) {
-        const int o = (k - 1) * xs;
-        b = (double(x[o]) - ta.acoef[k + 1] * b) * Z[(k - 1) * fs];
-        x[o] = T(b / size2d);
+    for (int k0 = N - 1; k0 >= 0; k0 -= kTC) {
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            cur[t] = nxt[t];
+            const int kk = k0 - kTC - t;
+            nxt[t] = kk >= 0 ? Z[kk * fs] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            const int kk = k0 - t;
+            if (kk >= 0) {
+                if (kk == N - 1) b = m0 ? 0.0 : double(x[kk * xs]) * cur[t];
+                else b = (double(x[kk * xs]) - ta.acoef[kk + 2] * b) * cur[t];
+                x[kk * xs] = T(b / size2d);
+            }
+        }
     }
 }
 
@@ -883,7 +941,7 @@ int solve_t(Ctx *c, double step, bool halos) {
         DYN_LAUNCH_CHECK();
         const int pad = 1;
         const size_t shy = y_pass_lds(pp.nj, pp.nk2, sizeof(T));
-        if (pp.fused_y) {
+        if (fused_shape(c, DYN_TUNE_PRESSURE_JPASS) == 1) {
             Timed _t(c, "p2_ypass", pts);
             hipLaunchKernelGGL(k_ypass<T>, dim3(nkx), dim3(ypass_threads<T>()), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
             DYN_LAUNCH_CHECK();
@@ -1050,7 +1108,7 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     DYN_LAUNCH_CHECK();
     const int pad = 1;
     const size_t shy = y_pass_lds(pp.nj, pp.nk2, sizeof(T));
-    if (pp.fused_y) {
+    if (fused_shape(c, DYN_TUNE_PRESSURE_JPASS) == 1) {
         Timed _t(c, "adj_ypass", pts);
         hipLaunchKernelGGL((k_ypass<T, true>), dim3(nkx), dim3(ypass_threads<T>()), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
         DYN_LAUNCH_CHECK();
