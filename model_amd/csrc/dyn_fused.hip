@@ -579,6 +579,7 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         for (int f = 0; f < NFS; f++) bot[f] = __shfl(tf[f], km1);
         const T qrb = MOIST ? __shfl(qrt, km1) : T(0);
         __syncthreads();
+        T oth = T(0), oqv = T(0), oqc = T(0), oqr = T(0);  // this level's final values (active levels)
         if (active) {
             T south[NFS];
 #pragma unroll
@@ -592,7 +593,8 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             {  // theta (advection.cpp:152-168)
                 T tend = -dtx * (east[0] - west[0]) - dty * (nf[0] - south[0]) - DTZ * (rw1 * tf[0] - rw0 * bot[0]) * odr -
                          DTZ * zbase(c.tb);
-                st<T>(a.out[3], p, mth + step * tend);
+                oth = mth + step * tend;
+                st<T>(a.out[3], p, oth);
             }
             if (MOIST) {  // advection.cpp:197-253
                 T tq = -dtx * (east[1] - west[1]) - dty * (nf[1] - south[1]) - DTZ * (rw1 * tf[1] - rw0 * bot[1]) * odr -
@@ -609,8 +611,37 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
                 st<T>(a.out[4], p, qv);
                 st<T>(a.out[5], p, qc);
                 st<T>(a.out[6], p, qrv);
+                oqv = qv;
+                oqc = qc;
+                oqr = qrv;
                 st<T>(a.vt, p, vtk);
                 if (k == 1) st<T>(a.vt, p - 1, vtk);
+            }
+        }
+        // the lid of upper_lower_boundaries (boundaries.cpp:304-312) for this column: level 0 takes level 1,
+        // NZ-1 takes NZ-2.  These are the final values of an interior column (no sponge or periodic copy
+        // touches it); the boundary epilogue redoes the columns it rewrites whole.  The sources of the
+        // periodic copy are left to it: halo column 0 receives column NX-6's values from before the lid.
+        {
+            const T t1 = __shfl(oth, 1), tz = __shfl(oth, NZ - 2);
+            T v1 = T(0), vz = T(0), c1 = T(0), cz = T(0), r1 = T(0), rz = T(0);
+            if (MOIST) {
+                v1 = __shfl(oqv, 1);
+                vz = __shfl(oqv, NZ - 2);
+                c1 = __shfl(oqc, 1);
+                cz = __shfl(oqc, NZ - 2);
+                r1 = __shfl(oqr, 1);
+                rz = __shfl(oqr, NZ - 2);
+            }
+            if (j < a.jh && i >= 6 && i < g.NX - 6 && (k == 0 || k == NZ - 1)) {
+                const unsigned pl = (unsigned)i * NYNZ + (unsigned)(jc * NZ + k);
+                const bool bot_lvl = k == 0;
+                st<T>(a.out[3], pl, bot_lvl ? t1 : tz);
+                if (MOIST) {
+                    st<T>(a.out[4], pl, bot_lvl ? v1 : vz);
+                    st<T>(a.out[5], pl, bot_lvl ? c1 : cz);
+                    st<T>(a.out[6], pl, bot_lvl ? r1 : rz);
+                }
             }
         }
 #pragma unroll
@@ -798,8 +829,11 @@ __device__ __forceinline__ void post_uvw_col(const Geo &g, const Cols &c, const 
 // sponge/periodic/lid for THP (and QVP, QCP, QRP with zero_moisture outside the advected box); only the
 // columns a sponge row, a ghost row or the periodic copy touches are rewritten whole, the others get the
 // two lid levels
+// lid_interior: also set the lid of the interior columns (the stage advection already did after a stage;
+// after Kessler it must be redone)
 template <typename T, bool MOIST>
-__device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, const PostArgs &a, bool clip, int jb, int isrc) {
+__device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, const PostArgs &a, bool clip, bool lid_interior, int jb,
+                                                 int isrc) {
     const int k = threadIdx.x;
     const int j = jb * blockDim.y + threadIdx.y;
     if (j >= g.NY) return;
@@ -810,7 +844,7 @@ __device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, co
     const bool full = coef >= T(0) || j < 3 || j >= g.NY - 3 || isrc < 6 || isrc >= g.NX - 6;
     constexpr int NFLD = MOIST ? 4 : 1;
     if (!full) {  // interior column: lid only (values at 1 and NZ-2 are final already)
-        if (j >= 1 && (k == 0 || k == NZ - 1)) {
+        if (lid_interior && j >= 1 && (k == 0 || k == NZ - 1)) {
             int ks = k == 0 ? 1 : NZ - 2;
 #pragma unroll
             for (int f = 0; f < NFLD; f++) {
@@ -848,8 +882,8 @@ __global__ void __launch_bounds__(256) k_post_uvw(Geo g, Cols c, PostArgs a, T s
     post_uvw_col<T>(g, c, a, step, pgf, blockIdx.x, blockIdx.y + 3);
 }
 template <typename T, bool MOIST>
-__global__ void __launch_bounds__(256) k_post_scalars(Geo g, Cols c, PostArgs a, bool clip) {
-    post_scalars_col<T, MOIST>(g, c, a, clip, blockIdx.x, blockIdx.y + 3);
+__global__ void __launch_bounds__(256) k_post_scalars(Geo g, Cols c, PostArgs a, bool clip, bool lid_interior) {
+    post_scalars_col<T, MOIST>(g, c, a, clip, lid_interior, blockIdx.x, blockIdx.y + 3);
 }
 
 // ---- batched over the members of an ensemble: one block per work item of the concatenated work list ----
@@ -911,7 +945,7 @@ __global__ void __launch_bounds__(256) k_ens_post(const EnsMember *mem, int nmem
     const PostArgs a = ens_post_args(e, ro);
     const int local = item - e.post0, nb = (e.g.NY + 3) / 4;
     post_uvw_col<T>(e.g, e.c, a, step, pgf, local % nb, local / nb + 3);
-    if (scalars) post_scalars_col<T, false>(e.g, e.c, a, false, local % nb, local / nb + 3);
+    if (scalars) post_scalars_col<T, false>(e.g, e.c, a, false, !pgf, local % nb, local / nb + 3);
 }
 
 // advance_outer's memcpy(M <- P) for u, v, w, theta of every member; work items are 1024-element chunks
@@ -955,8 +989,12 @@ int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip) {
     DYN_LAUNCH_CHECK();
     if (scalars) {
         Timed _t(c, "post_scalars", (double)c->npts);
-        if (c->g.microphysics) hipLaunchKernelGGL((k_post_scalars<T, true>), grd, blk, 0, c->stream, c->geo, c->cols, a, clip);
-        else hipLaunchKernelGGL((k_post_scalars<T, false>), grd, blk, 0, c->stream, c->geo, c->cols, a, clip);
+        // after a stage the advection kernel has set the interior lids; at the end of the step (no
+        // gradient force) Kessler has changed the interior, so they are redone
+        const bool lid_interior = !pgf;
+        if (c->g.microphysics)
+            hipLaunchKernelGGL((k_post_scalars<T, true>), grd, blk, 0, c->stream, c->geo, c->cols, a, clip, lid_interior);
+        else hipLaunchKernelGGL((k_post_scalars<T, false>), grd, blk, 0, c->stream, c->geo, c->cols, a, clip, lid_interior);
         DYN_LAUNCH_CHECK();
     }
     return DYN_OK;
