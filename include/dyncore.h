@@ -144,8 +144,12 @@ int dyn_tune_get(dyn_ctx *ctx, int param);
  * value set into all three buffers of that prognostic.  It recomputes each segment from its snapshot
  * and runs the transposed kernels backwards; upwind signs are those of the recorded trajectory.  The
  * context is left at the final state again.  Dry contexts only (microphysics 0); the trajectory memory
- * is 12 fields per snapshot plus 12 fields per step of one segment. */
+ * is 12 fields per snapshot plus 12 fields per step of one segment.  The record pass keeps the stage
+ * states of the last segment, so that segment is never recomputed: with every >= nsteps the reverse
+ * sweep recomputes nothing.  every = 0 picks the longest segment whose memory fits in half of the free
+ * device memory; dyn_adjoint_interval returns the segment length in use. */
 int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every);
+int dyn_adjoint_interval(dyn_ctx *ctx);
 int dyn_adjoint_backward(dyn_ctx *ctx, const double *seed_u, const double *seed_v, const double *seed_w, const double *seed_th,
                          double *grad_u, double *grad_v, double *grad_w, double *grad_th);
 int dyn_adjoint_release(dyn_ctx *ctx);

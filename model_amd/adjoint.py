@@ -33,9 +33,13 @@ def final_fields(core: DynCore):
     return tuple(core.get_field(f, ids.ROLE_CUR) for f in FIELDS)
 
 
-def record(core: DynCore, nsteps: int, every: int) -> None:
-    """Run nsteps from the context's state, keeping snapshots every `every` steps."""
+def record(core: DynCore, nsteps: int, every: int) -> int:
+    """Run nsteps from the context's state, keeping snapshots every `every` steps (0: the longest
+    interval that fits in half of the free device memory).  Returns the interval in use."""
     _check(core.L.dyn_adjoint_record(core.h, nsteps, every))
+    n = core.L.dyn_adjoint_interval(core.h)
+    _check(min(n, 0))
+    return n
 
 
 def backward(core: DynCore, seeds):

@@ -28,6 +28,7 @@ EXPORTS = [
     "dyn_advance_inner", "dyn_advance_outer", "dyn_micro_advance_inner", "dyn_micro_advance_outer",
     "dyn_step_reference", "dyn_step", "dyn_step_fused", "dyn_algorithmic_bytes_per_point", "dyn_timing", "dyn_timing_report",
     "dyn_stream", "dyn_tune_set", "dyn_tune_get", "dyn_adjoint_record", "dyn_adjoint_backward", "dyn_adjoint_release",
+    "dyn_adjoint_interval",
     "dyn_ensemble_create", "dyn_ensemble_destroy", "dyn_ensemble_synchronize", "dyn_ensemble_step", "dyn_ensemble_step_eager",
     "dyn_ensemble_set_shape", "dyn_ensemble_autotune",
 ]
@@ -87,6 +88,7 @@ def load():
     L.dyn_adjoint_record.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.dyn_adjoint_backward.argtypes = [ctypes.c_void_p] + [_dp] * 8
     L.dyn_adjoint_release.argtypes = [ctypes.c_void_p]
+    L.dyn_adjoint_interval.argtypes = [ctypes.c_void_p]
     L.dyn_ensemble_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     for fn in ("dyn_ensemble_destroy", "dyn_ensemble_synchronize"):
         getattr(L, fn).argtypes = [ctypes.c_void_p]

@@ -43,9 +43,10 @@ __device__ __forceinline__ T i3(T pm1, T p0, T p1, T p2, T sign) {
 }
 template <typename T>
 __device__ __forceinline__ T i2(T p0, T p1) { return T(0.5) * (p1 + p0); }
-// thread -> point (i, j, k) of the whole array, k fastest (every lane busy whatever NZ is)
+// thread -> point (i, j, k) of the whole array, k fastest (every lane busy whatever NZ is); each XCD
+// takes a contiguous range of points, so the i-neighbours a stencil reads sit behind its own L2
 __device__ __forceinline__ bool point_of(const Geo &g, int &i, int &j, int &k) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= (long long)g.NX * g.NYNZ) return false;
     i = (int)(q / g.NYNZ);
     const int r = (int)(q - (long long)i * g.NYNZ);
@@ -70,12 +71,13 @@ struct AdvAdjArgs {
 // (abar + a) Phi'(sign(abar + a)) returns (abar + a) dPhi'/d(operand) to the advected field and
 // (Phi_bar + Phi') da/d(velocity) to the advecting velocity.  Pass 1 (k_adj_faces) evaluates, once per
 // face, for the nine face kinds (u, v, theta on east, north and top faces):
-//   Aw = fh (abar + a), As = Aw sign(abar + a), G = fh (Phi_bar + Phi');
+//   Aw = fh (abar + a), s = sign(abar + a) (one byte), G = fh (Phi_bar + Phi');
 // pass 2 (k_adj_gather) gathers every term that reaches a point, the interpolant's derivative being
-// c_d + sign u_d (centred and upwind coefficients of operand d).
+// c_d + s u_d (centred and upwind coefficients of operand d).
 enum { FEU = 0, FEV, FET, FNU, FNV, FNT, FZU, FZV, FZT, NFACE };
 struct AdjFaces {
-    void *f[3 * NFACE];  // [kind][Aw, As, G], each a full field indexed by the face's low cell
+    void *f[2 * NFACE];         // [kind][Aw, G], each a full field indexed by the face's low cell
+    signed char *sgn[NFACE];    // [kind] upwind sign of the face
 };
 
 template <typename T>
@@ -117,10 +119,9 @@ __global__ void __launch_bounds__(256) k_adj_faces(Geo g, Cols c, AdvAdjArgs<T> 
     };
     const long long p = X(i, j, k);
     auto put = [&](int kind, T fh, T wt, T gg) {
-        const T aw = fh * wt;
-        ((T *)fa.f[3 * kind])[p] = aw;
-        ((T *)fa.f[3 * kind + 1])[p] = aw * sgn(wt);
-        ((T *)fa.f[3 * kind + 2])[p] = fh * gg;
+        ((T *)fa.f[2 * kind])[p] = fh * wt;
+        ((T *)fa.f[2 * kind + 1])[p] = fh * gg;
+        fa.sgn[kind][p] = wt > T(0) ? 1 : -1;
     };
     const T dtx = T(g.dtx), dty = T(g.dty);
     const bool krow = k >= 1 && k <= NZ - 2, jrow = j >= jl && j < jh, icol = i >= il && i < ih;
@@ -189,7 +190,11 @@ __global__ void __launch_bounds__(256) k_adj_gather(Geo g, Cols c, AdvAdjArgs<T>
     const int NZ = g.NZ, il = a.il, ih = a.ih, jl = a.jl, jh = a.jh;
     if (i < il - 3 || i >= ih + 3 || j < jl - 3 || j >= jh + 3) return;
     const AdjCommon<T> m{g, c, a, step};
-    auto F = [&](int kind, int q, int ii, int jj, int kk) -> T { return ((const T *)fa.f[3 * kind + q])[IX(ii, jj, kk)]; };
+    // q = 0: Aw; q = 2: G; S: the face's upwind sign
+    auto F = [&](int kind, int q, int ii, int jj, int kk) -> T { return ((const T *)fa.f[2 * kind + q / 2])[IX(ii, jj, kk)]; };
+    auto S = [&](int kind, int ii, int jj, int kk) -> T { return T(fa.sgn[kind][IX(ii, jj, kk)]); };
+    // Aw (c + s u) of a face for operand coefficients c, u
+    auto stencil = [&](int kind, int ii, int jj, int kk, T cc, T uu) -> T { return F(kind, 0, ii, jj, kk) * (cc + S(kind, ii, jj, kk) * uu); };
     // i5 = sum_d (c6[d] + sign * u5[d]) p_d and i3 = sum_d (c4[d] + sign * u3[d]) p_d
     const T c6[6] = {T(1.0 / 60.0), -T(2.0 / 15.0), T(37.0 / 60.0), T(37.0 / 60.0), -T(2.0 / 15.0), T(1.0 / 60.0)};
     const T u5[6] = {T(1.0 / 60.0), -T(5.0 / 60.0), T(10.0 / 60.0), -T(10.0 / 60.0), T(5.0 / 60.0), -T(1.0 / 60.0)};
@@ -202,9 +207,9 @@ __global__ void __launch_bounds__(256) k_adj_gather(Geo g, Cols c, AdvAdjArgs<T>
     if (jrow && krow) {
         for (int e = max(i - 3, il - 1); e <= min(i + 2, ih - 1); e++) {
             const int d = i - e + 2;
-            gu += F(FEU, 0, e, j, k) * c6[d] + F(FEU, 1, e, j, k) * u5[d];
-            gv += F(FEV, 0, e, j, k) * c6[d] + F(FEV, 1, e, j, k) * u5[d];
-            gth += F(FET, 0, e, j, k) * c6[d] + F(FET, 1, e, j, k) * u5[d];
+            gu += stencil(FEU, e, j, k, c6[d], u5[d]);
+            gv += stencil(FEV, e, j, k, c6[d], u5[d]);
+            gth += stencil(FET, e, j, k, c6[d], u5[d]);
             if (e == i - 1 || e == i) gu += T(0.5) * F(FEU, 2, e, j, k);
         }
     }
@@ -225,9 +230,9 @@ __global__ void __launch_bounds__(256) k_adj_gather(Geo g, Cols c, AdvAdjArgs<T>
     if (icol && krow) {
         for (int f = max(j - 3, jl - 1); f <= min(j + 2, jh - 1); f++) {
             const int d = j - f + 2;
-            gu += F(FNU, 0, i, f, k) * c6[d] + F(FNU, 1, i, f, k) * u5[d];
-            gv += F(FNV, 0, i, f, k) * c6[d] + F(FNV, 1, i, f, k) * u5[d];
-            gth += F(FNT, 0, i, f, k) * c6[d] + F(FNT, 1, i, f, k) * u5[d];
+            gu += stencil(FNU, i, f, k, c6[d], u5[d]);
+            gv += stencil(FNV, i, f, k, c6[d], u5[d]);
+            gth += stencil(FNT, i, f, k, c6[d], u5[d]);
             if (f == j - 1 || f == j) gv += T(0.5) * F(FNV, 2, i, f, k);
         }
     }
@@ -250,9 +255,9 @@ __global__ void __launch_bounds__(256) k_adj_gather(Geo g, Cols c, AdvAdjArgs<T>
                 }
             } else {
                 const int d = k - kf + 1;
-                gu += F(FZU, 0, i, j, kf) * c4[d] + F(FZU, 1, i, j, kf) * u3[d];
-                gv += F(FZV, 0, i, j, kf) * c4[d] + F(FZV, 1, i, j, kf) * u3[d];
-                gth += F(FZT, 0, i, j, kf) * c4[d] + F(FZT, 1, i, j, kf) * u3[d];
+                gu += stencil(FZU, i, j, kf, c4[d], u3[d]);
+                gv += stencil(FZV, i, j, kf, c4[d], u3[d]);
+                gth += stencil(FZT, i, j, kf, c4[d], u3[d]);
             }
         }
     }
@@ -430,6 +435,33 @@ __global__ void k_zero(T *__restrict__ a, long long n) {
     if (p < n) a[p] = T(0);
 }
 
+// up to 12 equal-sized device copies in one launch (grid.y = copy index), 16 bytes per lane and
+// 4-byte words for the tail; `words` = bytes / 4
+struct CopyList {
+    const void *src[12];
+    void *dst[12];
+};
+__global__ void __launch_bounds__(256) k_copy_list(CopyList cl, long long words) {
+    const uint4 *s = (const uint4 *)cl.src[blockIdx.y];
+    uint4 *d = (uint4 *)cl.dst[blockIdx.y];
+    const long long n16 = words / 4;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n16; q += (long long)gridDim.x * blockDim.x) d[q] = s[q];
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)(words - n16 * 4))
+        ((unsigned *)d)[n16 * 4 + threadIdx.x] = ((const unsigned *)s)[n16 * 4 + threadIdx.x];
+}
+int launch_copy_list(Ctx *c, const CopyList &cl, int n, size_t bytes) {
+    for (int q = 0; q < n; q++)
+        if (((uintptr_t)cl.src[q] | (uintptr_t)cl.dst[q]) % 16 != 0 || bytes % 4 != 0) {
+            set_error("adjoint copy: buffers must be 16-byte aligned and sizes whole words");
+            return DYN_E_ARG;
+        }
+    const long long words = (long long)(bytes / 4);
+    const long long blocks = std::min<long long>((words / 4 + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_copy_list, dim3((unsigned)std::max<long long>(blocks, 1), (unsigned)n), dim3(256), 0, c->stream, cl, words);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
 }  // namespace
 
 // Trajectory storage and adjoint buffers of one context (allocated by dyn_adjoint_record).
@@ -438,9 +470,12 @@ struct AdjState {
     size_t field_bytes = 0;
     std::vector<void *> snap;  // snapshot s, field f, role r at snap[(s * 4 + f) * 3 + r]; last = the final state
     std::vector<void *> rec;   // segment step n, stage st, field f at rec[(n * 3 + st) * 4 + f]
+    std::vector<void *> slabs; // allocations snap and rec are carved from
+    int rec_seg = -1;          // segment whose stage states rec holds (-1: none)
     void *adj[kAdjFields][3] = {{nullptr}};
     void *pihat = nullptr, *r1 = nullptr, *r2 = nullptr;
-    void *faces[27] = {nullptr};  // per-face adjoint weights of the advection transpose
+    void *faces[18] = {nullptr};  // per-face adjoint weights of the advection transpose (Aw, G per kind)
+    void *face_sgn[9] = {nullptr};  // per-face upwind signs (one byte each)
     double *stage_dbl = nullptr;  // host-transfer staging (double)
     int rec_base = 0;             // first recorded step of the segment in the record
 };
@@ -451,8 +486,7 @@ void adj_free(Ctx *c) {
     AdjState *s = c->adj;
     if (!s) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (void *p : s->snap) (void)hipFree(p);
-    for (void *p : s->rec) (void)hipFree(p);
+    for (void *p : s->slabs) (void)hipFree(p);
     for (auto &f : s->adj)
         for (void *p : f)
             if (p) (void)hipFree(p);
@@ -461,36 +495,51 @@ void adj_free(Ctx *c) {
         if (p) (void)hipFree(p);
     for (void *p : s->faces)
         if (p) (void)hipFree(p);
+    for (void *p : s->face_sgn)
+        if (p) (void)hipFree(p);
     delete s;
     c->adj = nullptr;
 }
 
-int alloc_fields(std::vector<void *> &v, size_t count, size_t bytes) {
-    v.assign(count, nullptr);
-    for (auto &p : v) DYN_HIP(hipMalloc(&p, bytes));
+// `count` fields of `bytes` each carved from one allocation, 256-byte aligned
+int alloc_fields(AdjState *s, std::vector<void *> &v, size_t count, size_t bytes) {
+    const size_t stride = (bytes + 255) / 256 * 256;
+    void *slab = nullptr;
+    if (hipMalloc(&slab, stride * count) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("dyn_adjoint_record: not enough device memory for the trajectory (" + std::to_string(stride * count >> 20) +
+                  " MiB); use a shorter checkpoint interval");
+        return DYN_E_HIP;
+    }
+    s->slabs.push_back(slab);
+    v.resize(count);
+    for (size_t q = 0; q < count; q++) v[q] = (char *)slab + q * stride;
     return DYN_OK;
 }
 
 int copy_state(Ctx *c, void *const *dst_or_null, void *const *src_or_null, bool to_snapshot) {
     AdjState *s = c->adj;
+    CopyList cl{};
     for (int f = 0; f < kAdjFields; f++)
         for (int r = 0; r < 3; r++) {
             void *prog = c->prog[kAdjIds[f]][r];
             void *snap = to_snapshot ? dst_or_null[f * 3 + r] : src_or_null[f * 3 + r];
-            if (to_snapshot) DYN_HIP(hipMemcpyAsync(snap, prog, s->field_bytes, hipMemcpyDeviceToDevice, c->stream));
-            else DYN_HIP(hipMemcpyAsync(prog, snap, s->field_bytes, hipMemcpyDeviceToDevice, c->stream));
+            cl.src[f * 3 + r] = to_snapshot ? prog : snap;
+            cl.dst[f * 3 + r] = to_snapshot ? snap : prog;
         }
-    return DYN_OK;
+    return launch_copy_list(c, cl, kAdjFields * 3, s->field_bytes);
 }
 
 // stage hook of the recomputation: the current buffers at the start of every stage
 int record_hook(Ctx *c, int stage, void *arg) {
     const int n = *(const int *)arg;
     AdjState *s = c->adj;
-    for (int f = 0; f < kAdjFields; f++)
-        DYN_HIP(hipMemcpyAsync(s->rec[((size_t)n * 3 + stage) * 4 + f], c->prog[kAdjIds[f]][DYN_ROLE_CUR], s->field_bytes,
-                               hipMemcpyDeviceToDevice, c->stream));
-    return DYN_OK;
+    CopyList cl{};
+    for (int f = 0; f < kAdjFields; f++) {
+        cl.src[f] = c->prog[kAdjIds[f]][DYN_ROLE_CUR];
+        cl.dst[f] = s->rec[((size_t)n * 3 + stage) * 4 + f];
+    }
+    return launch_copy_list(c, cl, kAdjFields, s->field_bytes);
 }
 
 template <typename T>
@@ -560,8 +609,9 @@ int stage_adjoint(Ctx *c, int n, int stage, double step) {
     a.jh = NY - 3;
     {
         AdjFaces fa;
-        static_assert(sizeof(fa.f) == sizeof(s->faces), "face array count");
+        static_assert(sizeof(fa.f) == sizeof(s->faces) && sizeof(fa.sgn) == sizeof(s->face_sgn), "face array count");
         memcpy(fa.f, s->faces, sizeof(fa.f));
+        for (int k = 0; k < NFACE; k++) fa.sgn[k] = (signed char *)s->face_sgn[k];
         Timed _t(c, "adj_advect", (double)c->npts);
         hipLaunchKernelGGL(k_adj_faces<T>, point_grid(g), dim3(256), 0, c->stream, g, c->cols, a, fa, T(step));
         DYN_LAUNCH_CHECK();
@@ -633,14 +683,18 @@ int backward_t(Ctx *c, const double *const *seed, double *const *grad) {
     const int nseg = (s->nsteps + s->every - 1) / s->every;
     for (int seg = nseg - 1; seg >= 0; seg--) {
         const int n0 = seg * s->every, n1 = std::min(s->nsteps, n0 + s->every);
-        int rc = copy_state(c, nullptr, &s->snap[(size_t)seg * 12], false);
-        if (rc) return rc;
-        s->rec_base = n0;
-        for (int n = n0; n < n1; n++) {
-            int local = n - n0;
-            rc = step_with_hook(c, record_hook, &local);
+        int rc = DYN_OK;
+        if (s->rec_seg != seg) {  // the record pass keeps the last segment: recompute only the others
+            rc = copy_state(c, nullptr, &s->snap[(size_t)seg * 12], false);
             if (rc) return rc;
+            for (int n = n0; n < n1; n++) {
+                int local = n - n0;
+                rc = step_with_hook(c, record_hook, &local);
+                if (rc) return rc;
+            }
+            s->rec_seg = seg;
         }
+        s->rec_base = n0;
         for (int n = n1 - 1; n >= n0; n--) {
             rc = step_adjoint<T>(c, n);
             if (rc) return rc;
@@ -682,19 +736,33 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
         set_error("dyn_adjoint_record: the adjoint covers the dry step (microphysics option 0)");
         return DYN_E_UNSUPPORTED;
     }
-    if (nsteps <= 0 || every <= 0) {
-        set_error("dyn_adjoint_record: nsteps and every must be positive");
+    if (nsteps <= 0 || every < 0) {
+        set_error("dyn_adjoint_record: nsteps must be positive and every non-negative");
         return DYN_E_ARG;
     }
     adj_free(c);
     AdjState *s = new AdjState();
     c->adj = s;
     s->nsteps = nsteps;
-    s->every = std::min(every, nsteps);
     s->field_bytes = c->elt * c->npts;
+    const size_t fstride = (s->field_bytes + 255) / 256 * 256;
+    if (every == 0) {
+        // longest segment whose records (12 fields a step) and snapshots (12 fields each) fit in half of
+        // the free device memory, less the ~40 fields of adjoint and face buffers allocated below
+        size_t freeb = 0, totalb = 0;
+        DYN_HIP(hipMemGetInfo(&freeb, &totalb));
+        const double budget = 0.5 * (double)freeb - 40.0 * (double)fstride;
+        every = nsteps;
+        while (every > 1) {
+            const int nseg = (nsteps + every - 1) / every;
+            if ((double)(every + nseg + 1) * 12.0 * (double)fstride <= budget) break;
+            every = (every + 1) / 2;
+        }
+    }
+    s->every = std::min(every, nsteps);
     const int nseg = (nsteps + s->every - 1) / s->every;
-    int rc = alloc_fields(s->snap, (size_t)(nseg + 1) * 12, s->field_bytes);
-    if (!rc) rc = alloc_fields(s->rec, (size_t)s->every * 12, s->field_bytes);
+    int rc = alloc_fields(s, s->snap, (size_t)(nseg + 1) * 12, s->field_bytes);
+    if (!rc) rc = alloc_fields(s, s->rec, (size_t)s->every * 12, s->field_bytes);
     if (rc) {
         adj_free(c);
         return rc;
@@ -705,16 +773,21 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
     DYN_HIP(hipMalloc(&s->r1, s->field_bytes));
     DYN_HIP(hipMalloc(&s->r2, s->field_bytes));
     for (auto &p : s->faces) DYN_HIP(hipMalloc(&p, s->field_bytes));
+    for (auto &p : s->face_sgn) DYN_HIP(hipMalloc(&p, c->npts));
     DYN_HIP(hipMalloc(&s->stage_dbl, sizeof(double) * c->npts));
-    // forward pass with a snapshot every `every` steps and one of the final state
+    // forward pass with a snapshot every `every` steps and one of the final state; the last segment's
+    // stage states are recorded on the way, so the reverse sweep starts without a recomputation
+    const int last0 = (nseg - 1) * s->every;
     for (int n = 0; n < nsteps; n++) {
         if (n % s->every == 0) {
             rc = copy_state(c, &s->snap[(size_t)(n / s->every) * 12], nullptr, true);
             if (rc) return rc;
         }
-        rc = step_with_hook(c, nullptr, nullptr);
+        int local = n - last0;
+        rc = n >= last0 ? step_with_hook(c, record_hook, &local) : step_with_hook(c, nullptr, nullptr);
         if (rc) return rc;
     }
+    s->rec_seg = nseg - 1;
     rc = copy_state(c, &s->snap[(size_t)nseg * 12], nullptr, true);
     if (rc) return rc;
     DYN_HIP(hipStreamSynchronize(c->stream));
@@ -739,6 +812,19 @@ int dyn_adjoint_backward(dyn_ctx *ctx, const double *seed_u, const double *seed_
     const double *seed[kAdjFields] = {seed_u, seed_v, seed_w, seed_th};
     double *grad[kAdjFields] = {grad_u, grad_v, grad_w, grad_th};
     return c->prec == DYN_FP32 ? backward_t<float>(c, seed, grad) : backward_t<double>(c, seed, grad);
+}
+
+int dyn_adjoint_interval(dyn_ctx *ctx) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) {
+        set_error("null context");
+        return DYN_E_ARG;
+    }
+    if (!c->adj) {
+        set_error("dyn_adjoint_interval: no recorded trajectory");
+        return DYN_E_STATE;
+    }
+    return c->adj->every;
 }
 
 int dyn_adjoint_release(dyn_ctx *ctx) {

@@ -50,13 +50,8 @@ __device__ __forceinline__ void st(void *base, unsigned idx, T v) {
     *(T *)((char *)base + (size_t)(idx * (unsigned)sizeof(T))) = v;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs.  Tile t = jblock + nbj * ichunk; hardware block
-// b is given a tile so that each XCD owns a contiguous run of tiles: neighbouring j-blocks of one i-chunk
-// (which share halo rows) then sit behind the same L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-    const int q = n / 8, rem = n % 8, x = b % 8;
-    return x * q + min(x, rem) + b / 8;
-}
+// Tile t = jblock + nbj * ichunk, placed by xcd_tile: neighbouring j-blocks of one i-chunk (which share
+// halo rows) sit behind the same L2.
 
 struct StageArgs {
     const void *cur[7];   // U V W TH QV QC QR (current)

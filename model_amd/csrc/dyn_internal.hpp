@@ -117,6 +117,12 @@ __device__ __forceinline__ int ens_member(const EnsMember *mem, int nmem, int it
     }
     return lo;
 }
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8): tile index of block b
+// of n such that each XCD takes one contiguous run of tiles, so neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n / 8, rem = n % 8, x = b % 8;
+    return x * q + min(x, rem) + b / 8;
+}
 struct AdjState;  // trajectory and adjoint buffers of the hand-written adjoint (dyn_adjoint.hip)
 
 struct Ctx {

@@ -86,3 +86,25 @@ def test_torch_autograd_wrapper():
     ref = backward(core2, [Wu, Wv, None, None])
     for t, r in zip(x0, ref):
         assert np.array_equal(t.grad.numpy(), r)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    """Every checkpoint interval (recomputed segments, the record pass's own last segment, the whole
+    trajectory kept, the automatic interval) gives the same gradient bit for bit, also on a second
+    backward sweep over the same record."""
+    c = _case()
+    Wu, Wv = vorticity_weights(c)
+    core = DynCore(c)
+    x0 = _initial(c)
+    ref = None
+    for every in (1, 4, 6, 0):
+        set_initial(core, x0)
+        used = record(core, 6, every)
+        assert used == (6 if every == 0 else every)
+        for sweep in range(2):
+            grads = backward(core, [Wu, Wv, None, None])
+            if ref is None:
+                ref = grads
+            for f, a, b in zip(FIELDS, ref, grads):
+                assert np.array_equal(a, b), (every, sweep, f)
+    core.close()
