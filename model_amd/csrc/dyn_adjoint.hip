@@ -43,10 +43,9 @@ __device__ __forceinline__ T i3(T pm1, T p0, T p1, T p2, T sign) {
 }
 template <typename T>
 __device__ __forceinline__ T i2(T p0, T p1) { return T(0.5) * (p1 + p0); }
-// thread -> point (i, j, k) of the whole array, k fastest (every lane busy whatever NZ is); each XCD
-// takes a contiguous range of points, so the i-neighbours a stencil reads sit behind its own L2
+// thread -> point (i, j, k) of the whole array, k fastest (every lane busy whatever NZ is)
 __device__ __forceinline__ bool point_of(const Geo &g, int &i, int &j, int &k) {
-    const long long q = (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long long)g.NX * g.NYNZ) return false;
     i = (int)(q / g.NYNZ);
     const int r = (int)(q - (long long)i * g.NYNZ);

@@ -108,3 +108,4 @@ def test_checkpoint_interval_does_not_change_the_gradient():
             for f, a, b in zip(FIELDS, ref, grads):
                 assert np.array_equal(a, b), (every, sweep, f)
     core.close()
+
