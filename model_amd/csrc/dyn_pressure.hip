@@ -20,6 +20,7 @@ namespace {
 
 
 constexpr int kThreads = 256;
+constexpr int kThreadsX = 512;  // the real-transform passes along i (more loads in flight per CU)
 constexpr size_t kLdsBudget = 64 * 1024;
 
 // exp(sign * 2 pi i * e / n) from the forward table (cos, -sin)
@@ -767,7 +768,7 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T *__restrict__ UP, const T *__restrict__ VP,
+__global__ void __launch_bounds__(kThreadsX) k_div_rfft_x(Geo g, Cols c, const T *__restrict__ UP, const T *__restrict__ VP,
                                                          const T *__restrict__ WP, T *__restrict__ spec, Radix rad, const double *tw,
                                                          int nk2, int kc, double step, const T *__restrict__ src) {
     extern __shared__ unsigned char smem[];
@@ -775,7 +776,7 @@ __global__ void __launch_bounds__(kThreads) k_div_rfft_x(Geo g, Cols c, const T 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_irfft_x_out(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
+__global__ void __launch_bounds__(kThreadsX) k_irfft_x_out(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
                                                           const double *tw, int nk2, int kc) {
     extern __shared__ unsigned char smem[];
     irfft_x_out_block<T>(g, spec, PI, rad, tw, nk2, kc, blockIdx.x, blockIdx.y, smem);
@@ -792,7 +793,7 @@ __global__ void __launch_bounds__(ypass_threads<T>()) k_ypass(T *__restrict__ sp
 // concatenates the members' (row, level chunk) or (kx) items -------------------------------------------
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_ens_div_rfft_x(const EnsMember *mem, int nmem, EnsRoles roles, int total, double step) {
+__global__ void __launch_bounds__(kThreadsX) k_ens_div_rfft_x(const EnsMember *mem, int nmem, EnsRoles roles, int total, double step) {
     extern __shared__ unsigned char smem[];
     {
         const int item = blockIdx.x;
@@ -805,7 +806,7 @@ __global__ void __launch_bounds__(kThreads) k_ens_div_rfft_x(const EnsMember *me
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_ens_irfft_x_out(const EnsMember *mem, int nmem, int total) {
+__global__ void __launch_bounds__(kThreadsX) k_ens_irfft_x_out(const EnsMember *mem, int nmem, int total) {
     extern __shared__ unsigned char smem[];
     {
         const int item = blockIdx.x;
@@ -907,7 +908,7 @@ int x_level_chunk(const PressurePlan &pp, const Geo &g, const Radix &rx, size_t 
     int kc = (int)((kLdsBudget - 2 * (size_t)pp.ni * elt) / (2 * (size_t)pp.ni * elt)) * 2;
     int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
     kc = std::max(2, std::min(kc, want));
-    while (kc > 2 && max_butterflies(rx, kc / 2, kThreads) > kMaxBX) kc -= 2;
+    while (kc > 2 && max_butterflies(rx, kc / 2, kThreadsX) > kMaxBX) kc -= 2;
     return kc;
 }
 
@@ -934,7 +935,7 @@ int solve_t(Ctx *c, double step, bool halos) {
         dim3 gx(g.NY, (pp.nk2 + kc - 1) / kc);
         {
             Timed _t(c, "p1_div_rfft_x", pts);
-            hipLaunchKernelGGL(k_div_rfft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
+            hipLaunchKernelGGL(k_div_rfft_x<T>, gx, dim3(kThreadsX), shx, c->stream, g, c->cols, (const T *)c->prog[DYN_U][2],
                                (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kc, step,
                                (const T *)nullptr);
         }
@@ -969,7 +970,7 @@ int solve_t(Ctx *c, double step, bool halos) {
         }
         {
             Timed _t(c, "p5_irfft_x_out", pts);
-            hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kc);
+            hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kc);
         }
         DYN_LAUNCH_CHECK();
         return DYN_OK;
@@ -1102,7 +1103,7 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     dim3 gx(g.NY, (pp.nk2 + kc - 1) / kc);
     {
         Timed _t(c, "adj_rfft_x", pts);
-        hipLaunchKernelGGL(k_div_rfft_x<T>, gx, dim3(kThreads), shx, c->stream, g, c->cols, (const T *)nullptr, (const T *)nullptr,
+        hipLaunchKernelGGL(k_div_rfft_x<T>, gx, dim3(kThreadsX), shx, c->stream, g, c->cols, (const T *)nullptr, (const T *)nullptr,
                            (const T *)nullptr, spec, rx, pp.twx, pp.nk2, kc, step, (const T *)r1);
     }
     DYN_LAUNCH_CHECK();
@@ -1128,7 +1129,7 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     }
     {
         Timed _t(c, "adj_irfft_x", pts);
-        hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreads), shx, c->stream, g, (const T *)spec, r2, rx, pp.twx, pp.nk2, kc);
+        hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)spec, r2, rx, pp.twx, pp.nk2, kc);
     }
     DYN_LAUNCH_CHECK();
     {
@@ -1270,7 +1271,7 @@ int launch_pressure_ens(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &r
     {
         const int grid = tot_px;
         Timed _t(c0, "ens_p1_div_rfft_x", 0.0);
-        hipLaunchKernelGGL(k_ens_div_rfft_x<float>, dim3(grid), dim3(kThreads), lds_x, c0->stream, d, nmem, ro, tot_px, step);
+        hipLaunchKernelGGL(k_ens_div_rfft_x<float>, dim3(grid), dim3(kThreadsX), lds_x, c0->stream, d, nmem, ro, tot_px, step);
         DYN_LAUNCH_CHECK();
     }
     {
@@ -1282,7 +1283,7 @@ int launch_pressure_ens(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &r
     {
         const int grid = tot_px;
         Timed _t(c0, "ens_p5_irfft_x_out", 0.0);
-        hipLaunchKernelGGL(k_ens_irfft_x_out<float>, dim3(grid), dim3(kThreads), lds_x, c0->stream, d, nmem, tot_px);
+        hipLaunchKernelGGL(k_ens_irfft_x_out<float>, dim3(grid), dim3(kThreadsX), lds_x, c0->stream, d, nmem, tot_px);
         DYN_LAUNCH_CHECK();
     }
     return DYN_OK;
