@@ -903,10 +903,13 @@ size_t x_pass_lds(int ni, int kc, size_t elt) { return (2 * (size_t)ni * (kc / 2
 size_t y_pass_lds(int nj, int nk2, size_t elt) {
     return 2 * (size_t)nj * (nk2 + 1) * elt + 8 * (2 * (size_t)nk2 + 1) + 2 * (size_t)nj * elt + 8;
 }
-// level pairs per block of the i passes: enough blocks to fill the chip, the tile within the LDS budget
-int x_level_chunk(const PressurePlan &pp, const Geo &g, const Radix &rx, size_t elt) {
+// levels per block of the i passes (even: two levels per complex transform), within the LDS budget and
+// the butterflies a thread holds.  One grid alone wants enough blocks to fill the chip (about 512 rows x
+// chunks); a batch of members has blocks to spare, so there every row takes all its levels at once
+// (`fill` false), which keeps the row segments long.
+int x_level_chunk(const PressurePlan &pp, const Geo &g, const Radix &rx, size_t elt, bool fill = true) {
     int kc = (int)((kLdsBudget - 2 * (size_t)pp.ni * elt) / (2 * (size_t)pp.ni * elt)) * 2;
-    int want = std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1);
+    int want = fill ? std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1) : (pp.nk2 + 1) & ~1;
     kc = std::max(2, std::min(kc, want));
     while (kc > 2 && max_butterflies(rx, kc / 2, kThreadsX) > kMaxBX) kc -= 2;
     return kc;
@@ -1241,7 +1244,7 @@ bool pressure_ens_fill(Ctx *c, EnsMember &e, size_t &lds_x, size_t &lds_y) {
     PressurePlan &pp = c->pp;
     if (pp.ni % 2 != 0 || !pp.fused_y || c->prec != DYN_FP32) return false;
     if (!factorize(pp.ni, e.rx) || !factorize(pp.nj, e.ry)) return false;
-    e.kcx = x_level_chunk(pp, c->geo, e.rx, sizeof(float));
+    e.kcx = x_level_chunk(pp, c->geo, e.rx, sizeof(float), false);
     e.ni = pp.ni;
     e.nj = pp.nj;
     e.nk2 = pp.nk2;
