@@ -127,6 +127,7 @@ def ensemble_bench(steps, warmup):
     pts = sum(c.interior_points for c in cases)
     ens = Ensemble(members)
     shape = ens.autotune()
+    groups = ens.groups
     ens.step(warmup)
     ens.sync()
     t0 = time.perf_counter()
@@ -150,7 +151,8 @@ def ensemble_bench(steps, warmup):
             "gridpoints_per_step": pts, "steps": steps, "ms_per_step": round(t_ens * 1e3 / steps, 4),
             "value": round(pts * steps / t_ens, 1), "unit": "gridpoint-steps/s",
             "members_one_by_one": round(pts * steps / t_seq, 1), "batched_over_one_by_one": round(t_seq / t_ens, 3),
-            "shape": {"momentum_tj": shape[0], "momentum_ti": shape[1], "scalar_tj": shape[2], "scalar_ti": shape[3]}}
+            "shape": {"momentum_tj": shape[0], "momentum_ti": shape[1], "scalar_tj": shape[2], "scalar_ti": shape[3]},
+            "stream_groups": groups}
 
 
 def adjoint_bench(steps=360, every=20):

@@ -172,6 +172,12 @@ int dyn_ensemble_set_shape(dyn_ensemble *ens, const int *shape);
 /* times every shape of the two stage kernels on the members' grids and keeps the fastest (written to
  * shape_out[4] when not NULL); the model state is unchanged */
 int dyn_ensemble_autotune(dyn_ensemble *ens, int *shape_out);
+/* splits the members into ngroups groups of about equal point counts, each advanced by the same batched
+ * launches on a stream of its own (forked from and joined back into the first member's stream, one
+ * branch per group in the captured graph), so that different passes of different groups overlap.
+ * Results are unchanged.  dyn_ensemble_groups returns the count in force (min(4, members) after create). */
+int dyn_ensemble_set_groups(dyn_ensemble *ens, int ngroups);
+int dyn_ensemble_groups(dyn_ensemble *ens);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,7 @@ EXPORTS = [
     "dyn_stream", "dyn_tune_set", "dyn_tune_get", "dyn_adjoint_record", "dyn_adjoint_backward", "dyn_adjoint_release",
     "dyn_adjoint_interval",
     "dyn_ensemble_create", "dyn_ensemble_destroy", "dyn_ensemble_synchronize", "dyn_ensemble_step", "dyn_ensemble_step_eager",
-    "dyn_ensemble_set_shape", "dyn_ensemble_autotune",
+    "dyn_ensemble_set_shape", "dyn_ensemble_autotune", "dyn_ensemble_set_groups", "dyn_ensemble_groups",
 ]
 
 
@@ -96,6 +96,8 @@ def load():
         getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_ensemble_set_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     L.dyn_ensemble_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    L.dyn_ensemble_set_groups.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dyn_ensemble_groups.argtypes = [ctypes.c_void_p]
     L.dyn_stream.restype = ctypes.c_void_p
     _lib = L
     return L
@@ -221,6 +223,15 @@ class Ensemble:
         arr = (ctypes.c_int * 4)()
         _check(self.L.dyn_ensemble_autotune(self.h, arr))
         return tuple(arr)
+
+    def set_groups(self, n):
+        _check(self.L.dyn_ensemble_set_groups(self.h, n))
+
+    @property
+    def groups(self):
+        n = self.L.dyn_ensemble_groups(self.h)
+        _check(min(n, 0))
+        return n
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

@@ -3,6 +3,11 @@
 // the members' work items are concatenated and each block finds its member in a device-resident member
 // table.  A step is captured once into a hipGraph (two steps: the buffer rotation has period 2).
 //
+// The members can be split into groups of about equal point counts (dyn_ensemble_set_groups): each group
+// runs the same batched passes over its own members on its own stream, so the passes of one group (say
+// the LDS-bound pressure transforms) overlap those of another (the stencils); the captured graph holds one
+// branch per group.
+//
 // All members must share the precision and NZ and be dry (microphysics 0).  The members stay ordinary
 // contexts: their field accessors work between ensemble steps after dyn_ensemble_synchronize.
 #include <algorithm>
@@ -13,6 +18,20 @@
 #include "dyn_internal.hpp"
 
 namespace dyn {
+
+// groups after dyn_ensemble_create: as many as the hardware queues a process gets by default (4), beyond
+// which further streams share queues
+constexpr int kDefaultGroups = 4;
+
+// members advanced by one stream: their table (offsets local to the group's launches) and work totals
+struct EnsGroup {
+    std::vector<int> idx;
+    std::vector<EnsMember> h;
+    EnsMember *d = nullptr;
+    int tot_mom = 0, tot_scal = 0, tot_post = 0, tot_px = 0, tot_py = 0, tot_cp = 0;
+    hipStream_t stream = nullptr;  // the first group runs on the first member's stream
+    hipEvent_t done = nullptr;
+};
 
 struct Ensemble {
     std::vector<Ctx *> mem;
@@ -26,6 +45,8 @@ struct Ensemble {
     size_t lds_x = 0, lds_y = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<EnsGroup> groups;
+    hipEvent_t fork = nullptr;
     Ctx *c0() const { return mem[0]; }
 };
 
@@ -37,11 +58,10 @@ EnsRoles roles_of(const Ensemble *e) {
     return r;
 }
 
-// work offsets for the current launch shapes; uploads the table (never during a capture)
-int layout(Ensemble *e) {
+// work offsets of `h` (members launched together) for the current launch shapes
+void offsets(const Ensemble *e, std::vector<EnsMember> &h, int &tmom, int &tscal, int &tpost, int &tpx, int &tpy, int &tcp) {
     int mom = 0, scal = 0, post = 0, px = 0, py = 0, cp = 0;
-    for (int m = 0; m < e->nmem; m++) {
-        EnsMember &t = e->h[m];
+    for (EnsMember &t : h) {
         const int nx = t.g.NX - 6, ny = t.g.NY - 6;
         t.nbj_mom = (ny + e->shape[0] - 1) / e->shape[0];
         t.nbj_scal = (ny + e->shape[2] - 1) / e->shape[2];
@@ -58,14 +78,69 @@ int layout(Ensemble *e) {
         t.cp0 = cp;
         cp += 4 * (int)(((long long)t.g.NX * t.g.NYNZ + 1023) / 1024);
     }
-    e->tot_mom = mom;
-    e->tot_scal = scal;
-    e->tot_post = post;
-    e->tot_px = px;
-    e->tot_py = py;
-    e->tot_cp = cp;
+    tmom = mom;
+    tscal = scal;
+    tpost = post;
+    tpx = px;
+    tpy = py;
+    tcp = cp;
+}
+
+// offsets of the whole ensemble (the autotuner's launches) and of every group; uploads the tables
+// (never during a capture)
+int layout(Ensemble *e) {
+    offsets(e, e->h, e->tot_mom, e->tot_scal, e->tot_post, e->tot_px, e->tot_py, e->tot_cp);
     DYN_HIP(hipMemcpy(e->d, e->h.data(), sizeof(EnsMember) * e->nmem, hipMemcpyHostToDevice));
+    for (EnsGroup &gr : e->groups) {
+        gr.h.clear();
+        for (int m : gr.idx) gr.h.push_back(e->h[m]);
+        offsets(e, gr.h, gr.tot_mom, gr.tot_scal, gr.tot_post, gr.tot_px, gr.tot_py, gr.tot_cp);
+        DYN_HIP(hipMemcpy(gr.d, gr.h.data(), sizeof(EnsMember) * gr.h.size(), hipMemcpyHostToDevice));
+    }
     return DYN_OK;
+}
+
+void free_groups(Ensemble *e) {
+    for (size_t g = 0; g < e->groups.size(); g++) {
+        EnsGroup &gr = e->groups[g];
+        if (gr.d) (void)hipFree(gr.d);
+        if (gr.done) (void)hipEventDestroy(gr.done);
+        if (g > 0 && gr.stream) (void)hipStreamDestroy(gr.stream);
+    }
+    e->groups.clear();
+}
+
+// split the members into n groups of about equal point counts (largest first, each to the lightest group)
+int make_groups(Ensemble *e, int n) {
+    free_groups(e);
+    n = std::max(1, std::min(n, e->nmem));
+    std::vector<int> order(e->nmem);
+    for (int m = 0; m < e->nmem; m++) order[m] = m;
+    auto pts = [&](int m) { return (long long)e->mem[m]->npts; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pts(a) > pts(b); });
+    e->groups.resize(n);
+    std::vector<long long> load(n, 0);
+    for (int m : order) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        e->groups[g].idx.push_back(m);
+        load[g] += pts(m);
+    }
+    for (int g = 0; g < n; g++) {
+        EnsGroup &gr = e->groups[g];
+        std::sort(gr.idx.begin(), gr.idx.end());
+        gr.stream = e->c0()->stream;
+        if (g > 0 && hipStreamCreateWithFlags(&gr.stream, hipStreamNonBlocking) != hipSuccess) {
+            gr.stream = nullptr;
+            set_error("dyn_ensemble: stream creation failed");
+            return DYN_E_HIP;
+        }
+        if (hipEventCreateWithFlags(&gr.done, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc(&gr.d, sizeof(EnsMember) * gr.idx.size()) != hipSuccess) {
+            set_error("dyn_ensemble: group allocation failed");
+            return DYN_E_HIP;
+        }
+    }
+    return layout(e);
 }
 
 void drop_graph(Ensemble *e) {
@@ -84,41 +159,71 @@ void advance_roles(Ensemble *e) {
     }
 }
 
-// one dry RK3 step of every member, in the fused form (fused_step of the single context)
+// runs f(c) with the context's stream replaced by `s` (the batched launchers take their stream from
+// the context they are given)
+template <typename F>
+int on_stream(Ctx *c, hipStream_t s, F &&f) {
+    hipStream_t own = c->stream;
+    c->stream = s;
+    const int rc = f(c);
+    c->stream = own;
+    return rc;
+}
+
+// one dry RK3 step of every member, in the fused form (fused_step of the single context); each group's
+// passes go to its own stream, forked from and joined back into the first member's stream
 int ens_step(Ensemble *e) {
     Ctx *c0 = e->c0();
     const double steps[] = {1. / 3., 0.5, 1.0};
     int rc;
+    const int ng = (int)e->groups.size();
+    if (ng > 1) {
+        DYN_HIP(hipEventRecord(e->fork, c0->stream));
+        for (int g = 1; g < ng; g++) DYN_HIP(hipStreamWaitEvent(e->groups[g].stream, e->fork, 0));
+    }
     for (int s = 0; s < 3; s++) {
         const EnsRoles ro = roles_of(e);
-        if ((rc = launch_ens_stage(c0, e->d, e->nmem, ro, steps[s], e->shape, e->tot_mom, e->tot_scal))) return rc;
-        if (e->batched_pressure) {
-            rc = launch_pressure_ens(c0, e->d, e->nmem, ro, steps[s], e->tot_px, e->tot_py, e->lds_x, e->lds_y);
-        } else {
-            // FP64 members: each member's own solve, on the ensemble's stream
-            for (Ctx *c : e->mem) {
-                hipStream_t own = c->stream;
-                c->stream = c0->stream;
-                rc = launch_solve_pressure(c, steps[s], false);
-                c->stream = own;
-                if (rc) break;
+        for (EnsGroup &gr : e->groups) {
+            const int nm = (int)gr.idx.size();
+            rc = on_stream(c0, gr.stream, [&](Ctx *c) {
+                return launch_ens_stage(c, gr.d, nm, ro, steps[s], e->shape, gr.tot_mom, gr.tot_scal);
+            });
+            if (rc) return rc;
+            if (e->batched_pressure) {
+                rc = on_stream(c0, gr.stream, [&](Ctx *c) {
+                    return launch_pressure_ens(c, gr.d, nm, ro, steps[s], gr.tot_px, gr.tot_py, e->lds_x, e->lds_y);
+                });
+            } else {
+                // FP64 members: each member's own solve, on the group's stream
+                for (int m : gr.idx) {
+                    rc = on_stream(e->mem[m], gr.stream, [&](Ctx *c) { return launch_solve_pressure(c, steps[s], false); });
+                    if (rc) break;
+                }
             }
+            if (rc) return rc;
+            rc = on_stream(c0, gr.stream, [&](Ctx *c) { return launch_ens_post(c, gr.d, nm, ro, steps[s], true, gr.tot_post); });
+            if (rc) return rc;
         }
-        if (rc) return rc;
-        if ((rc = launch_ens_post(c0, e->d, e->nmem, ro, steps[s], true, e->tot_post))) return rc;
         if (s < 2) advance_roles(e);
     }
     const EnsRoles ro = roles_of(e);
-    if ((rc = launch_ens_post(c0, e->d, e->nmem, ro, 0.0, false, e->tot_post))) return rc;
-    for (Ctx *c : e->mem) {
-        if (c->g.rayleigh_level >= c->g.nz) continue;
-        hipStream_t own = c->stream;
-        c->stream = c0->stream;
-        rc = launch_rayleigh(c, 1, c->g.nx - 1, 1, c->g.ny - 1, c->g.rayleigh_level);
-        c->stream = own;
+    for (EnsGroup &gr : e->groups) {
+        const int nm = (int)gr.idx.size();
+        rc = on_stream(c0, gr.stream, [&](Ctx *c) { return launch_ens_post(c, gr.d, nm, ro, 0.0, false, gr.tot_post); });
+        if (rc) return rc;
+        for (int m : gr.idx) {
+            Ctx *cm = e->mem[m];
+            if (cm->g.rayleigh_level >= cm->g.nz) continue;
+            rc = on_stream(cm, gr.stream, [&](Ctx *c) { return launch_rayleigh(c, 1, c->g.nx - 1, 1, c->g.ny - 1, c->g.rayleigh_level); });
+            if (rc) return rc;
+        }
+        rc = on_stream(c0, gr.stream, [&](Ctx *c) { return launch_ens_copy(c, gr.d, nm, ro, gr.tot_cp); });
         if (rc) return rc;
     }
-    if ((rc = launch_ens_copy(c0, e->d, e->nmem, ro, e->tot_cp))) return rc;
+    for (int g = 1; g < ng; g++) {
+        DYN_HIP(hipEventRecord(e->groups[g].done, e->groups[g].stream));
+        DYN_HIP(hipStreamWaitEvent(c0->stream, e->groups[g].done, 0));
+    }
     advance_roles(e);
     return DYN_OK;
 }
@@ -224,8 +329,14 @@ int dyn_ensemble_create(dyn_ctx *const *members, int n, dyn_ensemble **out) {
         set_error("dyn_ensemble_create: table allocation failed");
         rc = DYN_E_HIP;
     }
-    if (!rc) rc = layout(e);
+    if (!rc && hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) {
+        set_error("dyn_ensemble_create: event creation failed");
+        rc = DYN_E_HIP;
+    }
+    if (!rc) rc = make_groups(e, std::min(kDefaultGroups, n));
     if (rc) {
+        free_groups(e);
+        if (e->fork) (void)hipEventDestroy(e->fork);
         if (e->d) (void)hipFree(e->d);
         delete e;
         return rc;
@@ -239,9 +350,28 @@ int dyn_ensemble_destroy(dyn_ensemble *ens) {
     if (!e) return DYN_OK;
     (void)hipStreamSynchronize(e->c0()->stream);
     drop_graph(e);
+    free_groups(e);
+    if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->d) (void)hipFree(e->d);
     delete e;
     return DYN_OK;
+}
+
+int dyn_ensemble_set_groups(dyn_ensemble *ens, int ngroups) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(ens);
+    if (!e || ngroups < 1) {
+        set_error("dyn_ensemble_set_groups: need an ensemble and at least one group");
+        return DYN_E_ARG;
+    }
+    DYN_HIP(hipStreamSynchronize(e->c0()->stream));
+    drop_graph(e);
+    return make_groups(e, ngroups);
+}
+
+int dyn_ensemble_groups(dyn_ensemble *ens) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(ens);
+    if (!e) return DYN_E_ARG;
+    return (int)e->groups.size();
 }
 
 int dyn_ensemble_synchronize(dyn_ensemble *ens) {

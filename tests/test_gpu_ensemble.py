@@ -15,9 +15,10 @@ def _cases():
     return [configs.vortex_in_shear(nx=n, ny=m, nz=20, seed_offset=10 + k) for k, (n, m) in enumerate([(48, 64), (64, 48), (80, 80)])]
 
 
+@pytest.mark.parametrize("groups", [1, 2, 3], ids=["1-group", "2-groups", "3-groups"])
 @pytest.mark.parametrize("precision", [DYN_FP32, 0], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
-def test_ensemble_matches_members_alone(precision, graph):
+def test_ensemble_matches_members_alone(precision, graph, groups):
     cases = _cases()
     alone = [DynCore(c, precision=precision) for c in cases]
     for g in alone:
@@ -25,6 +26,9 @@ def test_ensemble_matches_members_alone(precision, graph):
         g.sync()
     members = [DynCore(c, precision=precision) for c in cases]
     ens = Ensemble(members)
+    assert ens.groups == min(4, len(members))
+    ens.set_groups(groups)
+    assert ens.groups == groups
     ens.step(5, graph=graph)
     ens.sync()
     for a, b in zip(alone, members):
