@@ -53,9 +53,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_form(case, precision, steps, warmup):
-    from model_amd.core import DynCore
+def time_form(case, precision, steps, warmup, stage_streams):
+    from model_amd.core import DYN_TUNE_STAGE_STREAMS, DynCore
     g = DynCore(case, precision=precision)
+    g.tune_set(DYN_TUNE_STAGE_STREAMS, stage_streams)
     g.step(warmup)
     g.sync()
     t0 = time.perf_counter()
@@ -66,10 +67,16 @@ def time_form(case, precision, steps, warmup):
 
 
 def kernel_profile(g, steps, fused):
-    """Per-kernel HIP-event timings of the same step sequence the graph replays, launched eagerly."""
+    """Per-kernel HIP-event timings of the same step sequence the graph replays, launched eagerly, with
+    the whole stage on one stream: the scalar half, which the step runs beside the pressure solve, would
+    otherwise share the GPU with the kernels it overlaps and its duration would not be its own."""
+    from model_amd.core import DYN_TUNE_STAGE_STREAMS
+    before = g.tune_get(DYN_TUNE_STAGE_STREAMS)
+    g.tune_set(DYN_TUNE_STAGE_STREAMS, 1)
     g.timing(1)
     (g.step_fused if fused else g.step_reference)(steps)
     g.sync()
+    g.tune_set(DYN_TUNE_STAGE_STREAMS, before)
     import ctypes
     buf = ctypes.create_string_buffer(1 << 16)
     g.timing_report(buf, len(buf))
@@ -215,6 +222,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint measurements")
+    ap.add_argument("--stage-streams", type=int, choices=[1, 2], default=2,
+                    help="2: a stage's scalar half runs on a second stream beside the pressure solve (default); "
+                         "1: the whole step on one stream (per-kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would shard); "
@@ -232,7 +242,7 @@ def main():
         [(args.form, DYN_FP32 if args.form == "fp32" else DYN_FP64)]
     handles = {}
     for name, prec in forms:
-        g, ms = time_form(case, prec, args.steps, args.warmup)
+        g, ms = time_form(case, prec, args.steps, args.warmup, args.stage_streams)
         res[name] = {"ms_per_step": round(ms, 4), "value": round(npts / (ms * 1e-3), 1)}
         handles[name] = g
         log(f"bench: {name} {ms:.3f} ms/step, {res[name]['value']:.4g} gridpoint-steps/s")
@@ -256,7 +266,8 @@ def main():
         "dtype": "f32" if main_form == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": "moist Kessler supercell, 256x256x64 (NX=262 incl. halo), dx=1 km, dz=250 m, dt=2 s",
-                   "nx": 256, "ny": 256, "nz": 64, "form": main_form, "step": "hipGraph replay of s_integrate_rk3"},
+                   "nx": 256, "ny": 256, "nz": 64, "form": main_form, "step": "hipGraph replay of s_integrate_rk3",
+                   "stage_streams": args.stage_streams},
         "forms": res,
         "fp32_over_fp64": round(res["fp32"]["value"] / res["fp64"]["value"], 3) if "fp64" in res and "fp32" in res else None,
         "adjoint_forward_ratio": None,

@@ -18,6 +18,7 @@ REFAPI_PATH = os.path.join(_HERE, "_build", "libdyncore_refapi.so")
 
 DYN_FP64, DYN_FP32 = 0, 1
 DYN_TUNE_MOMENTUM_TJ, DYN_TUNE_MOMENTUM_TI, DYN_TUNE_SCALAR_TJ, DYN_TUNE_SCALAR_TI, DYN_TUNE_PRESSURE_JPASS = 0, 1, 2, 3, 4
+DYN_TUNE_STAGE_STREAMS = 5
 ERRORS = {-1: "DYN_E_ARG", -2: "DYN_E_UNSUPPORTED", -3: "DYN_E_HIP", -4: "DYN_E_NUMERIC", -5: "DYN_E_STATE"}
 
 # the exported C symbols, as declared in include/dyncore.h

@@ -41,9 +41,9 @@ Timed::~Timed() {
 int pressure_coefficients(Ctx *c);
 bool fused_supported(const Ctx *c);
 int fused_prepare(Ctx *c);
-int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh);
+int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh, int parts);
 int launch_zero_moisture_shell(Ctx *c, int il, int ih, int jl, int jh);
-int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip);
+int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw);
 
 namespace {
 
@@ -195,6 +195,12 @@ int dyn_create(const dyn_grid *grid, dyn_ctx **out) {
         delete c;
         return DYN_E_HIP;
     }
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        set_error("second stream or its events could not be created");
+        return fail(DYN_E_HIP);
+    }
     size_t bytes = c->npts * c->elt;
     int nprog = g.microphysics ? 7 : 4;
     for (int f = 0; f < nprog; f++)
@@ -269,6 +275,9 @@ int dyn_destroy(dyn_ctx *ctx) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free(c->colhost);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DYN_OK;
@@ -594,22 +603,47 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     do {                           \
         if ((rc = (x))) return rc; \
     } while (0)
+    // The scalar half of a stage (theta and moisture advection, their boundary update) reads the
+    // current buffers only and writes buffers nothing else in the stage reads, so it runs on the second
+    // stream beside the pressure solve, which needs only the u, v, w forecasts of the momentum half.
+    // Started after the momentum kernel (the head of the stage's critical path), its blocks fill the
+    // CUs the latency-bound transforms leave idle.
+    const bool two = fused_shape(c, DYN_TUNE_STAGE_STREAMS) == 2;
+    auto on_side = [&](auto &&launch) {
+        if (!two) return launch();
+        hipStream_t own = c->stream;
+        c->stream = c->side;
+        const int r = launch();
+        c->stream = own;
+        return r;
+    };
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
         if (hook) RC(hook(c, s, arg));
-        RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3));
+        RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageMomentum));
+        if (two && (hipEventRecord(c->ev_fork, c->stream) != hipSuccess || hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess)) {
+            set_error("fused step: fork onto the second stream failed");
+            return DYN_E_HIP;
+        }
+        RC(on_side([&] { return launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageScalars); }));
+        // zero_moisture outside the advected box and the scalars' boundary update
+        RC(on_side([&] { return launch_post_stage(c, step, true, true, true, false); }));
         // the halo columns periodic_uvw_boundaries writes are read only by the divergence (which reads
         // UP(3) for UP(NX-3)); the boundary update overwrites them before anything else reads them
         RC(launch_solve_pressure(c, step, false));
-        // pressure gradient, zero_moisture outside the advected box and both boundary updates
-        RC(launch_post_stage(c, step, true, true, true));
+        // pressure gradient and the boundary update of u, v, w
+        RC(launch_post_stage(c, step, true, false, false, true));
+        if (two && (hipEventRecord(c->ev_join, c->side) != hipSuccess || hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess)) {
+            set_error("fused step: join of the second stream failed");
+            return DYN_E_HIP;
+        }
         if (moist && s < 2) RC(dyn_micro_advance_inner(ctx));
         if (s < 2) RC(dyn_advance_inner(ctx));
     }
     if (moist) RC(dyn_kessler(ctx, 1, NX - 1, 1, NY - 1));
     // apply_boundary_condition_microphysics then apply_boundary_condition (different fields, so one
     // kernel serves both; the moisture pointers swap only afterwards)
-    RC(launch_post_stage(c, 0.0, false, true, false));
+    RC(launch_post_stage(c, 0.0, false, true, false, true));
     if (moist) RC(dyn_micro_advance_outer(ctx));
     RC(dyn_rayleigh(ctx, 1, NX - 1, 1, NY - 1, c->g.rayleigh_level));
     RC(dyn_advance_outer(ctx));

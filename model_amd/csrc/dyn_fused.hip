@@ -713,7 +713,7 @@ int scalars_t(Ctx *c, const StageArgs &a0, T step, int ti) {
 }
 
 template <typename T, bool MOIST>
-int stage_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
+int stage_t(Ctx *c, double step, int il, int ih, int jl, int jh, int parts) {
     StageArgs a{};
     for (int f = 0; f < 7; f++) {
         a.cur[f] = c->prog[f][DYN_ROLE_CUR];
@@ -728,11 +728,13 @@ int stage_t(Ctx *c, double step, int il, int ih, int jl, int jh) {
     a.jh = jh;
     const int mtj = fused_shape(c, DYN_TUNE_MOMENTUM_TJ), mti = fused_shape(c, DYN_TUNE_MOMENTUM_TI);
     const int stj = fused_shape(c, DYN_TUNE_SCALAR_TJ), sti = fused_shape(c, DYN_TUNE_SCALAR_TI);
-    int rc;
-    if (mtj == 2) rc = momentum_t<T, 2, MOIST>(c, a, T(step), mti);
-    else if (mtj == 4) rc = momentum_t<T, 4, MOIST>(c, a, T(step), mti);
-    else rc = momentum_t<T, 8, MOIST>(c, a, T(step), mti);
-    if (rc) return rc;
+    int rc = DYN_OK;
+    if (parts & kStageMomentum) {
+        if (mtj == 2) rc = momentum_t<T, 2, MOIST>(c, a, T(step), mti);
+        else if (mtj == 4) rc = momentum_t<T, 4, MOIST>(c, a, T(step), mti);
+        else rc = momentum_t<T, 8, MOIST>(c, a, T(step), mti);
+    }
+    if (rc || !(parts & kStageScalars)) return rc;
     if (stj == 2) return scalars_t<T, 2, MOIST>(c, a, T(step), sti);
     if (stj == 4) return scalars_t<T, 4, MOIST>(c, a, T(step), sti);
     return scalars_t<T, 8, MOIST>(c, a, T(step), sti);
@@ -962,7 +964,7 @@ __global__ void __launch_bounds__(256) k_ens_copy(const EnsMember *mem, int nmem
 }
 
 template <typename T>
-int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip) {
+int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw) {
     PostArgs a{};
     for (int f = 0; f < 7; f++) a.f[f] = c->prog[f][DYN_ROLE_P];
     a.pi = c->pi;
@@ -977,11 +979,11 @@ int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip) {
     a.zjh = c->g.ny - 1;
     const int rows = 4;
     dim3 blk(KP, rows), grd((c->g.ny + rows - 1) / rows, c->g.nx - 6);
-    {
+    if (uvw) {
         Timed _t(c, "post_uvw", (double)c->npts);
         hipLaunchKernelGGL(k_post_uvw<T>, grd, blk, 0, c->stream, c->geo, c->cols, a, T(step), pgf);
+        DYN_LAUNCH_CHECK();
     }
-    DYN_LAUNCH_CHECK();
     if (scalars) {
         Timed _t(c, "post_scalars", (double)c->npts);
         // after a stage the advection kernel has set the interior lids; at the end of the step (no
@@ -1005,6 +1007,7 @@ bool fused_shape_ok(const Ctx *c, int param, int value) {
     if (param == DYN_TUNE_MOMENTUM_TJ || param == DYN_TUNE_SCALAR_TJ) return value == 2 || value == 4 || value == 8;
     if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
     if (param == DYN_TUNE_PRESSURE_JPASS) return value == 2 || (value == 1 && c->pp.fused_y);
+    if (param == DYN_TUNE_STAGE_STREAMS) return value == 1 || value == 2;
     return false;
 }
 
@@ -1017,6 +1020,7 @@ int fused_shape(const Ctx *c, int param) {
     case DYN_TUNE_MOMENTUM_TI:
     case DYN_TUNE_SCALAR_TI: return 16;
     case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : 2;
+    case DYN_TUNE_STAGE_STREAMS: return 2;
     default: return DYN_E_ARG;
     }
 }
@@ -1050,22 +1054,23 @@ int fused_prepare(Ctx *c) {
 
 // advect_uvw_velocity + advect_theta (+ advect_microphysics_cell and the interior zero_moisture)
 // over the box [il,ih) x [jl,jh)
-int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh) {
+int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh, int parts) {
     // the i-queues reach columns il-3 .. ih+2 and the planes rows jl-3 .. jh+2
     if (il < 3 || ih > c->g.nx - 3 || jl < 3 || jh > c->g.ny - 3 || il >= ih || jl >= jh) {
         set_error("fused stage: box outside [3, NX-3) x [3, NY-3)");
         return DYN_E_ARG;
     }
     const bool moist = c->g.microphysics != 0;
-    if (c->prec == DYN_FP32) return moist ? stage_t<float, true>(c, step, il, ih, jl, jh) : stage_t<float, false>(c, step, il, ih, jl, jh);
-    return moist ? stage_t<double, true>(c, step, il, ih, jl, jh) : stage_t<double, false>(c, step, il, ih, jl, jh);
+    if (c->prec == DYN_FP32)
+        return moist ? stage_t<float, true>(c, step, il, ih, jl, jh, parts) : stage_t<float, false>(c, step, il, ih, jl, jh, parts);
+    return moist ? stage_t<double, true>(c, step, il, ih, jl, jh, parts) : stage_t<double, false>(c, step, il, ih, jl, jh, parts);
 }
 
 // PGF (optional) and the boundary update of every prognostic after launch_fused_advect; stands for
 // dyn_pgf + zero_moisture's remainder + dyn_apply_bc + dyn_apply_bc_micro of one stage
-int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip) {
-    if (c->prec == DYN_FP32) return post_t<float>(c, step, pgf, scalars, clip);
-    return post_t<double>(c, step, pgf, scalars, clip);
+int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw) {
+    if (c->prec == DYN_FP32) return post_t<float>(c, step, pgf, scalars, clip, uvw);
+    return post_t<double>(c, step, pgf, scalars, clip, uvw);
 }
 
 // the rest of zero_moisture(il,ih,jl,jh) after launch_fused_advect on the box [3,NX-3) x [3,NY-3)

@@ -123,6 +123,9 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     const int q = n / 8, rem = n % 8, x = b % 8;
     return x * q + min(x, rem) + b / 8;
 }
+// the two halves of a fused stage (launch_fused_advect's `parts`)
+constexpr int kStageMomentum = 1, kStageScalars = 2;
+
 struct AdjState;  // trajectory and adjoint buffers of the hand-written adjoint (dyn_adjoint.hip)
 
 struct Ctx {
@@ -132,6 +135,10 @@ struct Ctx {
     size_t elt = 8;       // bytes per field element
     size_t npts = 0;      // NX*NY*NZ
     hipStream_t stream = nullptr;
+    // second stream of the fused step: the scalar half of a stage (advection + boundary update) runs on
+    // it beside the momentum half and the pressure solve, forked and joined by the two events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // prognostics: [field][role] device pointers; roles rotate by pointer swaps
     void *prog[7][3] = {{nullptr}};
     void *pi = nullptr, *vt = nullptr, *accrain = nullptr;

@@ -232,6 +232,28 @@ def test_fused_launch_shapes_bitwise(precision, shape):
         assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), ids.FIELD_NAMES[f]
 
 
+@pytest.mark.parametrize("precision,moist", [(DYN_FP32, 1), (DYN_FP32, 0), (0, 1)], ids=["fp32-moist", "fp32-dry", "fp64-moist"])
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_stage_on_two_streams_bitwise(precision, moist, graph):
+    """The scalar half of a stage on the second stream, beside the pressure solve, gives the bits of the
+    whole stage on one stream (all buffer roles and the fall speed)."""
+    from model_amd import core
+    c = configs.supercell(nx=40, ny=36, nz=30) if moist else configs.vortex_in_shear(nx=40, ny=36, nz=30)
+    a, b = DynCore(c, precision=precision), DynCore(c, precision=precision)
+    a.tune_set(core.DYN_TUNE_STAGE_STREAMS, 1)
+    assert b.tune_get(core.DYN_TUNE_STAGE_STREAMS) == 2
+    for g in (a, b):
+        (g.step if graph else g.step_fused)(5)
+        g.sync()
+    fields = (ids.U, ids.V, ids.W, ids.TH) + ((ids.QV, ids.QC, ids.QR) if moist else ())
+    for f in fields:
+        for role in (ids.ROLE_M, CUR, ids.ROLE_P):
+            assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), (ids.FIELD_NAMES[f], role)
+    if moist:
+        assert np.array_equal(a.get_field(ids.VT, 0), b.get_field(ids.VT, 0))
+        assert np.array_equal(a.get_field(ids.ACCRAIN), b.get_field(ids.ACCRAIN))
+
+
 @pytest.mark.parametrize("graph", [False, True], ids=["per-operator", "graph"])
 def test_odd_transform_length_against_oracle(graph):
     """ni = 15 and nj = 18 take the full-complex transform path (odd i length) of the pressure solve."""
