@@ -66,6 +66,20 @@ __global__ void k_copy_f(float *__restrict__ dst, const float *__restrict__ src,
     long long s = (long long)gridDim.x * blockDim.x;
     for (; p < n; p += s) dst[p] = src[p];
 }
+// advance_outer's M <- P copies of several fields in one launch (grid.y = field), 16 bytes per lane and
+// 4-byte words for the tail; `words` = bytes per field / 4
+struct FieldCopies {
+    const void *src[7];
+    void *dst[7];
+};
+__global__ void __launch_bounds__(256) k_copy_fields(FieldCopies fc, long long words) {
+    const uint4 *s = (const uint4 *)fc.src[blockIdx.y];
+    uint4 *d = (uint4 *)fc.dst[blockIdx.y];
+    const long long n16 = words / 4;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n16; q += (long long)gridDim.x * blockDim.x) d[q] = s[q];
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)(words - n16 * 4))
+        ((unsigned *)d)[n16 * 4 + threadIdx.x] = ((const unsigned *)s)[n16 * 4 + threadIdx.x];
+}
 __global__ void k_clear_flag(int *f) { *f = 0; }
 
 int device_copy(Ctx *c, void *dst, const void *src) {
@@ -74,6 +88,22 @@ int device_copy(Ctx *c, void *dst, const void *src) {
     Timed _t(c, "advance_copy", (double)n);
     if (c->prec == DYN_FP32) hipLaunchKernelGGL(k_copy_f, dim3(blocks), dim3(256), 0, c->stream, (float *)dst, (const float *)src, n);
     else hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, c->stream, (double *)dst, (const double *)src, n);
+    DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
+// M <- P of fields [f0, f1] in one launch (hipMalloc'd buffers: 256-byte aligned)
+int device_copy_fields(Ctx *c, int f0, int f1) {
+    FieldCopies fc{};
+    const int nf = f1 - f0 + 1;
+    for (int f = f0; f <= f1; f++) {
+        fc.src[f - f0] = c->prog[f][DYN_ROLE_P];
+        fc.dst[f - f0] = c->prog[f][DYN_ROLE_M];
+    }
+    const long long words = (long long)(c->npts * c->elt / 4);
+    const unsigned blocks = (unsigned)std::max<long long>(1, std::min<long long>((words / 4 + 255) / 256, 2048));
+    Timed _t(c, "advance_copy", (double)c->npts * nf);
+    hipLaunchKernelGGL(k_copy_fields, dim3(blocks, nf), dim3(256), 0, c->stream, fc, words);
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -644,9 +674,12 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     // apply_boundary_condition_microphysics then apply_boundary_condition (different fields, so one
     // kernel serves both; the moisture pointers swap only afterwards)
     RC(launch_post_stage(c, 0.0, false, true, false, true));
-    if (moist) RC(dyn_micro_advance_outer(ctx));
     RC(dyn_rayleigh(ctx, 1, NX - 1, 1, NY - 1, c->g.rayleigh_level));
-    RC(dyn_advance_outer(ctx));
+    // microphysics_advance then advance_outer: Rayleigh damping leaves the moisture fields alone, so
+    // the M <- P copies of every prognostic go in one launch after it
+    RC(device_copy_fields(c, DYN_U, moist ? DYN_QR : DYN_TH));
+    if (moist) RC(swap_roles(c, DYN_QV, DYN_QR));
+    RC(swap_roles(c, DYN_U, DYN_TH));
 #undef RC
     return DYN_OK;
 }
