@@ -598,6 +598,10 @@ constexpr int kTC = 8;
 // tables; the (0,0) column keeps its eliminated values in the FP64 scratch mean[2*kk] instead of the
 // storage type (pressure.cpp:837-871).  Level k = kk+1; acoef[1] = 0, so the first level needs no
 // special case.
+// The sweeps walk kTC levels at a time.  Full chunks are straight-line code: the chunk's values, its
+// acoef entries and the next chunk's table entries (clamped indices, so every load is unconditional) are
+// all issued before the chain starts, and only the last partial chunk takes guarded steps.  Branchy
+// chunks made the compiler wait on every load as soon as it was issued, prefetch included.
 template <typename T>
 __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, double *mean) {
     const int N = ta.nk - 2;
@@ -605,50 +609,69 @@ __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, 
     const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
     const bool m0 = (kx == 0 && ky == 0);
-    double cur[kTC], nxt[kTC];
+    double cur[kTC], nxt[kTC], v[kTC];
 #pragma unroll
-    for (int t = 0; t < kTC; t++) nxt[t] = t < N ? Z[t * fs] : 0.0;
+    for (int t = 0; t < kTC; t++) nxt[t] = Z[min(t, N - 1) * fs];
     double p = 0.0;
-    for (int k0 = 0; k0 < N; k0 += kTC) {
+    int k0 = 0;
+    for (; k0 + kTC <= N; k0 += kTC) {
+        double ac[kTC];
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
             cur[t] = nxt[t];
-            const int kk = k0 + kTC + t;
-            nxt[t] = kk < N ? Z[kk * fs] : 0.0;
+            nxt[t] = Z[min(k0 + kTC + t, N - 1) * fs];
+            v[t] = double(x[(k0 + t) * xs]);
+            ac[t] = ta.acoef[k0 + t + 1];
         }
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
-            const int kk = k0 + t;
-            if (kk < N) {
-                p = (double(x[kk * xs]) / size2d - ta.acoef[kk + 1] * p) * cur[t];
-                if (m0) mean[2 * kk] = p;
-                else x[kk * xs] = T(p);
-            }
+            p = (v[t] / size2d - ac[t] * p) * cur[t];
+            v[t] = p;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) {
+            if (m0) mean[2 * (k0 + t)] = v[t];
+            else x[(k0 + t) * xs] = T(v[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = k0 + t;
+        if (kk < N) {
+            p = (double(x[kk * xs]) / size2d - ta.acoef[kk + 1] * p) * nxt[t];
+            if (m0) mean[2 * kk] = p;
+            else x[kk * xs] = T(p);
         }
     }
     if (m0) p = 0.0;  // OUT3D(0,0,NZ-2) = 0
     x[(N - 1) * xs] = T(p);
     // back substitution x_kk = y_kk - F_kk x_{kk+1} for kk = N-2 .. 0, walking down in chunks
 #pragma unroll
-    for (int t = 0; t < kTC; t++) {
-        const int kk = N - 2 - t;
-        nxt[t] = kk >= 0 ? F[kk * fs] : 0.0;
-    }
-    for (int k0 = N - 2; k0 >= 0; k0 -= kTC) {
+    for (int t = 0; t < kTC; t++) nxt[t] = F[max(N - 2 - t, 0) * fs];
+    k0 = N - 2;
+    for (; k0 - kTC + 1 >= 0; k0 -= kTC) {
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
             cur[t] = nxt[t];
-            const int kk = k0 - kTC - t;
-            nxt[t] = kk >= 0 ? F[kk * fs] : 0.0;
+            nxt[t] = F[max(k0 - kTC - t, 0) * fs];
+            const double xm = mean[2 * (k0 - t)], xv = double(x[(k0 - t) * xs]);
+            v[t] = m0 ? xm : xv;
         }
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
-            const int kk = k0 - t;
-            if (kk >= 0) {
-                const double y = m0 ? mean[2 * kk] : double(x[kk * xs]);
-                p = y - cur[t] * p;
-                x[kk * xs] = T(p);
-            }
+            p = v[t] - cur[t] * p;
+            v[t] = p;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) x[(k0 - t) * xs] = T(v[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = k0 - t;
+        if (kk >= 0) {
+            const double y = m0 ? mean[2 * kk] : double(x[kk * xs]);
+            p = y - nxt[t] * p;
+            x[kk * xs] = T(p);
         }
     }
 }
@@ -665,49 +688,64 @@ __device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs
     const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
     const bool m0 = (kx == 0 && ky == 0);
-    double cur[kTC], nxt[kTC];
+    double cur[kTC], nxt[kTC], v[kTC];
     // q_kk = x_kk - F_{kk-1} q_{kk-1}, kk = 1 .. N-1 (q_0 = x_0)
 #pragma unroll
-    for (int t = 0; t < kTC; t++) nxt[t] = t < N - 1 ? F[t * fs] : 0.0;
+    for (int t = 0; t < kTC; t++) nxt[t] = F[min(t, N - 2) * fs];
     double q = double(x[0]);
-    for (int k0 = 1; k0 < N; k0 += kTC) {
+    int k0 = 1;
+    for (; k0 + kTC <= N; k0 += kTC) {
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
             cur[t] = nxt[t];
-            const int kk = k0 + kTC + t;
-            nxt[t] = kk < N ? F[(kk - 1) * fs] : 0.0;
+            nxt[t] = F[min(k0 + kTC + t - 1, N - 2) * fs];
+            v[t] = double(x[(k0 + t) * xs]);
         }
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
-            const int kk = k0 + t;
-            if (kk < N) {
-                q = double(x[kk * xs]) - cur[t] * q;
-                x[kk * xs] = T(q);
-            }
+            q = v[t] - cur[t] * q;
+            v[t] = q;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) x[(k0 + t) * xs] = T(v[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = k0 + t;
+        if (kk < N) {
+            q = double(x[kk * xs]) - nxt[t] * q;
+            x[kk * xs] = T(q);
         }
     }
     // b_{N-1} = z_{N-1} q_{N-1} (0 for (0,0)); b_kk = z_kk (q_kk - a_{kk+2} b_{kk+1}), kk = N-2 .. 0
+    double b = m0 ? 0.0 : double(x[(N - 1) * xs]) * Z[(N - 1) * fs];
+    x[(N - 1) * xs] = T(b / size2d);
 #pragma unroll
-    for (int t = 0; t < kTC; t++) {
-        const int kk = N - 1 - t;
-        nxt[t] = kk >= 0 ? Z[kk * fs] : 0.0;
-    }
-    double b = 0.0;
-    for (int k0 = N - 1; k0 >= 0; k0 -= kTC) {
+    for (int t = 0; t < kTC; t++) nxt[t] = Z[max(N - 2 - t, 0) * fs];
+    k0 = N - 2;
+    for (; k0 - kTC + 1 >= 0; k0 -= kTC) {
+        double ac[kTC];
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
             cur[t] = nxt[t];
-            const int kk = k0 - kTC - t;
-            nxt[t] = kk >= 0 ? Z[kk * fs] : 0.0;
+            nxt[t] = Z[max(k0 - kTC - t, 0) * fs];
+            v[t] = double(x[(k0 - t) * xs]);
+            ac[t] = ta.acoef[k0 - t + 2];
         }
 #pragma unroll
         for (int t = 0; t < kTC; t++) {
-            const int kk = k0 - t;
-            if (kk >= 0) {
-                if (kk == N - 1) b = m0 ? 0.0 : double(x[kk * xs]) * cur[t];
-                else b = (double(x[kk * xs]) - ta.acoef[kk + 2] * b) * cur[t];
-                x[kk * xs] = T(b / size2d);
-            }
+            b = (v[t] - ac[t] * b) * cur[t];
+            v[t] = b / size2d;
+        }
+#pragma unroll
+        for (int t = 0; t < kTC; t++) x[(k0 - t) * xs] = T(v[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = k0 - t;
+        if (kk >= 0) {
+            b = (double(x[kk * xs]) - ta.acoef[kk + 2] * b) * nxt[t];
+            x[kk * xs] = T(b / size2d);
         }
     }
 }
