@@ -369,7 +369,53 @@ __global__ void __launch_bounds__(kThreads) k_ifft_x_out(Geo g, const T *__restr
 // -sin of 2 pi e / n).  Work item idx = tid + t*nthreads is butterfly jj = idx / nbatch of batch
 // b = idx % nbatch; that split is done once, and the per-pass index arithmetic uses shifts while the
 // running length Ns is a power of two (every pass of the radix-4/2-first plans before a 3 or 5).
-template <typename T, int MAXB>
+// One radix-R pass of the power-of-two plans of lds_fft_ip with R a template parameter: items past the
+// end read a valid slot (and are not written back), so the pass is straight-line code and every LDS
+// load is issued before the first is waited on.
+template <typename T, int R, int MAXB>
+__device__ __forceinline__ void fft_pass_pow2(T *re, T *im, int n, int Ns, int lg, int sn, int sb, const int *bq, const int *jq,
+                                              const T *twl, int sign) {
+    const int nbf = n / R;
+    const int stride = n / (Ns * R);
+    T xr[MAXB][R], xi[MAXB][R];
+    int oidx[MAXB];
+#pragma unroll
+    for (int t = 0; t < MAXB; t++) {
+        const bool ok = jq[t] < nbf;
+        const int jj = ok ? jq[t] : 0, b = bq[t];
+        const int jm = jj & (Ns - 1), jd = jj >> lg;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int e = (jj + r * nbf) * sn + b * sb;
+            xr[t][r] = re[e];
+            xi[t][r] = im[e];
+        }
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const int e = r * jm * stride;
+                const T wr = twl[2 * e], wi = sign < 0 ? twl[2 * e + 1] : -twl[2 * e + 1];
+                cmul(xr[t][r], xi[t][r], wr, wi);
+            }
+        }
+        butterfly(R, xr[t], xi[t], sign);
+        oidx[t] = ok ? (jd * Ns * R + jm) * sn + b * sb : -1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MAXB; t++) {
+        if (oidx[t] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                re[oidx[t] + r * Ns * sn] = xr[t][r];
+                im[oidx[t] + r * Ns * sn] = xi[t][r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int MAXB, bool POW2 = false>
 __device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, int sb, const T *twl, int sign) {
     const int n = rad.n;
     const int nthr = blockDim.x;
@@ -379,6 +425,16 @@ __device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, i
         const int idx = threadIdx.x + t * nthr;
         bq[t] = idx % nbatch;
         jq[t] = idx / nbatch;
+    }
+    if (POW2 && (n & (n - 1)) == 0) {  // power of two: radix-4 passes, then one radix-2 pass when log2 n is odd
+        int Ns = 1, lg = 0;
+        for (int s = 0; s < rad.nr; s++) {
+            if (rad.r[s] == 4) fft_pass_pow2<T, 4, MAXB>(re, im, n, Ns, lg, sn, sb, bq, jq, twl, sign);
+            else fft_pass_pow2<T, 2, MAXB>(re, im, n, Ns, lg, sn, sb, bq, jq, twl, sign);
+            Ns *= rad.r[s];
+            lg = 31 - __clz(Ns);
+        }
+        return;
     }
     int Ns = 1, lg = 0;
     for (int s = 0; s < rad.nr; s++) {
@@ -780,7 +836,7 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
         }
     }
     __syncthreads();
-    lds_fft_ip<T, ypass_maxb<T>()>(re, im, rady, nk2, P2, 1, twl, -1);
+    lds_fft_ip<T, ypass_maxb<T>(), true>(re, im, rady, nk2, P2, 1, twl, -1);
     if ((int)threadIdx.x < 2 * nj) {
         // one thread per real sequence: threads [0, nj) take the real parts, [nj, 2 nj) the imaginary parts
         const int part = (int)threadIdx.x >= nj, ky = (int)threadIdx.x - part * nj;
@@ -789,7 +845,7 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
         else thomas_part<T>(ta, kx, ky, x, 1, mean + part);
     }
     __syncthreads();
-    lds_fft_ip<T, ypass_maxb<T>()>(re, im, rady, nk2, P2, 1, twl, +1);
+    lds_fft_ip<T, ypass_maxb<T>(), true>(re, im, rady, nk2, P2, 1, twl, +1);
     {
         int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
         for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
