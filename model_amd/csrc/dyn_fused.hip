@@ -829,10 +829,9 @@ __device__ __forceinline__ void post_uvw_col(const Geo &g, const Cols &c, const 
 // lid_interior: also set the lid of the interior columns (the stage advection already did after a stage;
 // after Kessler it must be redone)
 template <typename T, bool MOIST>
-__device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, const PostArgs &a, bool clip, bool lid_interior, int jb,
+__device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, const PostArgs &a, bool clip, bool lid_interior, int j,
                                                  int isrc) {
     const int k = threadIdx.x;
-    const int j = jb * blockDim.y + threadIdx.y;
     if (j >= g.NY) return;
     const int NZ = g.NZ;
     const int kc = k < NZ ? k : NZ - 1;
@@ -880,7 +879,44 @@ __global__ void __launch_bounds__(256) k_post_uvw(Geo g, Cols c, PostArgs a, T s
 }
 template <typename T, bool MOIST>
 __global__ void __launch_bounds__(256) k_post_scalars(Geo g, Cols c, PostArgs a, bool clip, bool lid_interior) {
-    post_scalars_col<T, MOIST>(g, c, a, clip, lid_interior, blockIdx.x, blockIdx.y + 3);
+    post_scalars_col<T, MOIST>(g, c, a, clip, lid_interior, blockIdx.x * blockDim.y + threadIdx.y, blockIdx.y + 3);
+}
+
+// Source columns post_scalars_col rewrites whole when the interior lids are already set (after a
+// stage): rows j < rl and j >= NY - rh (sponge and ghost rows) for every isrc in [3, NX-3), then, on the
+// other rows, the six periodic source columns.  One wavefront per column, item = its index in that list.
+struct ShellCols {
+    int rl, rh, nb, ni;  // low rows, high rows, rows between, source columns (NX - 6)
+    int ncb;             // periodic source columns (6, or every column on a narrow grid)
+    int total;
+};
+inline ShellCols shell_cols(const Ctx *c) {
+    ShellCols s;
+    const int NY = c->g.ny, ni = c->g.nx - 6;
+    s.rl = std::min(NY, std::max(c->g.sponge_south, 3));
+    s.rh = std::min(NY - s.rl, std::max(c->g.sponge_north, 3));
+    s.nb = NY - s.rl - s.rh;
+    s.ni = ni;
+    s.ncb = ni <= 6 ? ni : 6;
+    s.total = (s.rl + s.rh) * ni + s.ncb * s.nb;
+    return s;
+}
+template <typename T, bool MOIST>
+__global__ void __launch_bounds__(256) k_post_scalars_shell(Geo g, Cols c, PostArgs a, bool clip, ShellCols s) {
+    const int item = blockIdx.x * blockDim.y + threadIdx.y;
+    if (item >= s.total) return;
+    const int na = (s.rl + s.rh) * s.ni;
+    int j, isrc;
+    if (item < na) {
+        const int r = item / s.ni;
+        isrc = 3 + item % s.ni;
+        j = r < s.rl ? r : g.NY - s.rh + (r - s.rl);
+    } else {
+        const int q = item - na, cb = q / s.nb;
+        j = s.rl + q % s.nb;
+        isrc = s.ni <= 6 ? 3 + cb : (cb < 3 ? 3 + cb : g.NX - 6 + (cb - 3));
+    }
+    post_scalars_col<T, MOIST>(g, c, a, clip, false, j, isrc);
 }
 
 // ---- batched over the members of an ensemble: one block per work item of the concatenated work list ----
@@ -942,7 +978,7 @@ __global__ void __launch_bounds__(256) k_ens_post(const EnsMember *mem, int nmem
     const PostArgs a = ens_post_args(e, ro);
     const int local = item - e.post0, nb = (e.g.NY + 3) / 4;
     post_uvw_col<T>(e.g, e.c, a, step, pgf, local % nb, local / nb + 3);
-    if (scalars) post_scalars_col<T, false>(e.g, e.c, a, false, !pgf, local % nb, local / nb + 3);
+    if (scalars) post_scalars_col<T, false>(e.g, e.c, a, false, !pgf, (local % nb) * blockDim.y + threadIdx.y, local / nb + 3);
 }
 
 // advance_outer's memcpy(M <- P) for u, v, w, theta of every member; work items are 1024-element chunks
@@ -989,9 +1025,17 @@ int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw) {
         // after a stage the advection kernel has set the interior lids; at the end of the step (no
         // gradient force) Kessler has changed the interior, so they are redone
         const bool lid_interior = !pgf;
-        if (c->g.microphysics)
+        if (!lid_interior) {  // only the sponge rows, ghost rows and periodic source columns change
+            const ShellCols sc = shell_cols(c);
+            const dim3 gs((unsigned)std::max(1, (sc.total + rows - 1) / rows));
+            if (c->g.microphysics)
+                hipLaunchKernelGGL((k_post_scalars_shell<T, true>), gs, blk, 0, c->stream, c->geo, c->cols, a, clip, sc);
+            else hipLaunchKernelGGL((k_post_scalars_shell<T, false>), gs, blk, 0, c->stream, c->geo, c->cols, a, clip, sc);
+        } else if (c->g.microphysics) {
             hipLaunchKernelGGL((k_post_scalars<T, true>), grd, blk, 0, c->stream, c->geo, c->cols, a, clip, lid_interior);
-        else hipLaunchKernelGGL((k_post_scalars<T, false>), grd, blk, 0, c->stream, c->geo, c->cols, a, clip, lid_interior);
+        } else {
+            hipLaunchKernelGGL((k_post_scalars<T, false>), grd, blk, 0, c->stream, c->geo, c->cols, a, clip, lid_interior);
+        }
         DYN_LAUNCH_CHECK();
     }
     return DYN_OK;
