@@ -808,7 +808,7 @@ __device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs
 
 // Q2 (fused): forward DFT along j, FP64 tridiagonal solve, backward DFT along j for one kx, with the
 // whole (j, k) slab of that kx resident in LDS
-template <typename T, bool TR>
+template <typename T, bool TR, bool POW2 = true>
 __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const Radix &rady, const double *twy, const ThomasArgs &ta,
                                             int pad, int kx, unsigned char *smem) {
     const int nj = rady.n, P2 = nk2 + pad;
@@ -836,7 +836,7 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
         }
     }
     __syncthreads();
-    lds_fft_ip<T, ypass_maxb<T>(), true>(re, im, rady, nk2, P2, 1, twl, -1);
+    lds_fft_ip<T, ypass_maxb<T>(), POW2>(re, im, rady, nk2, P2, 1, twl, -1);
     if ((int)threadIdx.x < 2 * nj) {
         // one thread per real sequence: threads [0, nj) take the real parts, [nj, 2 nj) the imaginary parts
         const int part = (int)threadIdx.x >= nj, ky = (int)threadIdx.x - part * nj;
@@ -845,7 +845,7 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
         else thomas_part<T>(ta, kx, ky, x, 1, mean + part);
     }
     __syncthreads();
-    lds_fft_ip<T, ypass_maxb<T>(), true>(re, im, rady, nk2, P2, 1, twl, +1);
+    lds_fft_ip<T, ypass_maxb<T>(), POW2>(re, im, rady, nk2, P2, 1, twl, +1);
     {
         int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
         for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
@@ -917,7 +917,8 @@ __global__ void __launch_bounds__(1024) k_ens_ypass(const EnsMember *mem, int nm
         const int item = blockIdx.x;
         const EnsMember &e = mem[ens_member(mem, nmem, item, &EnsMember::py0)];
         ThomasArgs ta{e.coskx, e.cosly, e.c.rhou, e.c.rhow, e.c.mu, e.c.mw, e.acoef, e.ffac, e.dx, e.dz, e.ni, e.nj, e.g.NZ, e.zpiv};
-        ypass_block<T, false>((T *)e.spec, e.nk2, e.ry, e.twy, ta, 1, item - e.py0, smem);
+        // the generic transform passes: the straight-line power-of-two ones need stack here
+        ypass_block<T, false, false>((T *)e.spec, e.nk2, e.ry, e.twy, ta, 1, item - e.py0, smem);
     }
 }
 
