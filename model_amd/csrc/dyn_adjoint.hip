@@ -22,7 +22,8 @@
 namespace dyn {
 namespace {
 
-#define IX(i, j, k) ((long long)(i) * g.NYNZ + (long long)(j) * g.NZ + (long long)(k))
+// element offsets are 32-bit: dyn_adjoint_record accepts fields of fewer than 2^31 points
+#define IX(i, j, k) ((unsigned)(i) * (unsigned)g.NYNZ + (unsigned)(j) * (unsigned)g.NZ + (unsigned)(k))
 
 constexpr int kAdjFields = 4;  // U V W TH
 const int kAdjIds[kAdjFields] = {DYN_U, DYN_V, DYN_W, DYN_TH};
@@ -45,10 +46,10 @@ template <typename T>
 __device__ __forceinline__ T i2(T p0, T p1) { return T(0.5) * (p1 + p0); }
 // thread -> point (i, j, k) of the whole array, k fastest (every lane busy whatever NZ is)
 __device__ __forceinline__ bool point_of(const Geo &g, int &i, int &j, int &k) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long long)g.NX * g.NYNZ) return false;
-    i = (int)(q / g.NYNZ);
-    const int r = (int)(q - (long long)i * g.NYNZ);
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)g.NX * (unsigned)g.NYNZ) return false;
+    i = (int)(q / (unsigned)g.NYNZ);
+    const int r = (int)(q - (unsigned)i * (unsigned)g.NYNZ);
     j = r / g.NZ;
     k = r - j * g.NZ;
     return true;
@@ -738,6 +739,10 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
     if (nsteps <= 0 || every < 0) {
         set_error("dyn_adjoint_record: nsteps must be positive and every non-negative");
         return DYN_E_ARG;
+    }
+    if (c->npts >= (size_t(1) << 31)) {
+        set_error("dyn_adjoint_record: the adjoint kernels take fields of fewer than 2^31 points");
+        return DYN_E_UNSUPPORTED;
     }
     adj_free(c);
     AdjState *s = new AdjState();
