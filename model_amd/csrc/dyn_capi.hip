@@ -655,18 +655,21 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
             set_error("fused step: fork onto the second stream failed");
             return DYN_E_HIP;
         }
-        RC(on_side([&] { return launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageScalars); }));
+        int rs = on_side([&] { return launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageScalars); });
         // zero_moisture outside the advected box and the scalars' boundary update
-        RC(on_side([&] { return launch_post_stage(c, step, true, true, true, false); }));
+        if (!rs) rs = on_side([&] { return launch_post_stage(c, step, true, true, true, false); });
         // the halo columns periodic_uvw_boundaries writes are read only by the divergence (which reads
         // UP(3) for UP(NX-3)); the boundary update overwrites them before anything else reads them
-        RC(launch_solve_pressure(c, step, false));
+        int rm = launch_solve_pressure(c, step, false);
         // pressure gradient and the boundary update of u, v, w
-        RC(launch_post_stage(c, step, true, false, false, true));
+        if (!rm) rm = launch_post_stage(c, step, true, false, false, true);
+        // the join comes first, also after a failed launch, so no work is left on the second stream
         if (two && (hipEventRecord(c->ev_join, c->side) != hipSuccess || hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess)) {
             set_error("fused step: join of the second stream failed");
             return DYN_E_HIP;
         }
+        RC(rs);
+        RC(rm);
         if (moist && s < 2) RC(dyn_micro_advance_inner(ctx));
         if (s < 2) RC(dyn_advance_inner(ctx));
     }
@@ -684,7 +687,8 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     return DYN_OK;
 }
 
-// the sequence dyn_step records: fused stages for the FP32 form, per-operator kernels for FP64
+// the sequence dyn_step records: the fused stages when NZ fits one wavefront, else the per-operator
+// kernels
 static int graph_step(dyn_ctx *ctx) {
     Ctx *c = as_ctx(ctx);
     if (fused_supported(c)) return fused_step(ctx);

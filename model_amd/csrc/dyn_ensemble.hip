@@ -170,17 +170,12 @@ int on_stream(Ctx *c, hipStream_t s, F &&f) {
     return rc;
 }
 
-// one dry RK3 step of every member, in the fused form (fused_step of the single context); each group's
-// passes go to its own stream, forked from and joined back into the first member's stream
-int ens_step(Ensemble *e) {
+// the launches of one dry RK3 step of every member: each group's passes on its own stream (roles
+// rotate on the host between the stages)
+int ens_launch_step(Ensemble *e) {
     Ctx *c0 = e->c0();
     const double steps[] = {1. / 3., 0.5, 1.0};
     int rc;
-    const int ng = (int)e->groups.size();
-    if (ng > 1) {
-        DYN_HIP(hipEventRecord(e->fork, c0->stream));
-        for (int g = 1; g < ng; g++) DYN_HIP(hipStreamWaitEvent(e->groups[g].stream, e->fork, 0));
-    }
     for (int s = 0; s < 3; s++) {
         const EnsRoles ro = roles_of(e);
         for (EnsGroup &gr : e->groups) {
@@ -220,10 +215,25 @@ int ens_step(Ensemble *e) {
         rc = on_stream(c0, gr.stream, [&](Ctx *c) { return launch_ens_copy(c, gr.d, nm, ro, gr.tot_cp); });
         if (rc) return rc;
     }
+    return DYN_OK;
+}
+
+// one dry RK3 step of every member, in the fused form (fused_step of the single context); each group's
+// passes go to its own stream, forked from and joined back into the first member's stream (the joins
+// follow even when a launch fails, so no work is left on a group's stream)
+int ens_step(Ensemble *e) {
+    Ctx *c0 = e->c0();
+    const int ng = (int)e->groups.size();
+    if (ng > 1) {
+        DYN_HIP(hipEventRecord(e->fork, c0->stream));
+        for (int g = 1; g < ng; g++) DYN_HIP(hipStreamWaitEvent(e->groups[g].stream, e->fork, 0));
+    }
+    const int rc = ens_launch_step(e);
     for (int g = 1; g < ng; g++) {
         DYN_HIP(hipEventRecord(e->groups[g].done, e->groups[g].stream));
         DYN_HIP(hipStreamWaitEvent(c0->stream, e->groups[g].done, 0));
     }
+    if (rc) return rc;
     advance_roles(e);
     return DYN_OK;
 }
