@@ -129,7 +129,8 @@ enum {
     DYN_TUNE_MOMENTUM_TI = 1, /* columns one momentum block marches along i: 2 .. 512 */
     DYN_TUNE_SCALAR_TJ = 2,   /* j-rows per block of the scalar (theta, moisture) stage kernel: 2, 4 or 8 */
     DYN_TUNE_SCALAR_TI = 3,   /* columns one scalar block marches along i: 2 .. 512 */
-    DYN_TUNE_PRESSURE_JPASS = 4, /* pressure j pass: 1 one LDS-resident kernel per kx (when it fits), 2 three kernels */
+    DYN_TUNE_PRESSURE_JPASS = 4, /* pressure j pass: 1 one LDS-resident kernel per kx (when the slab fits), 2 three
+                                    kernels, 3 two LDS-resident kernels per kx on half slabs (even NY, when they fit) */
     DYN_TUNE_STAGE_STREAMS = 5,  /* fused step: 2 runs a stage's scalar half on a second stream beside the
                                     pressure solve, 1 runs the whole stage on the context's stream */
     DYN_TUNE_COUNT = 6

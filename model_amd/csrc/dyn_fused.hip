@@ -1050,7 +1050,7 @@ bool fused_supported(const Ctx *c) { return c->g.nz <= KP && (double)c->npts * (
 bool fused_shape_ok(const Ctx *c, int param, int value) {
     if (param == DYN_TUNE_MOMENTUM_TJ || param == DYN_TUNE_SCALAR_TJ) return value == 2 || value == 4 || value == 8;
     if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
-    if (param == DYN_TUNE_PRESSURE_JPASS) return value == 2 || (value == 1 && c->pp.fused_y);
+    if (param == DYN_TUNE_PRESSURE_JPASS) return value == 2 || (value == 1 && c->pp.fused_y) || (value == 3 && c->pp.half_y);
     if (param == DYN_TUNE_STAGE_STREAMS) return value == 1 || value == 2;
     return false;
 }
@@ -1067,7 +1067,7 @@ int fused_shape(const Ctx *c, int param) {
     case DYN_TUNE_SCALAR_TJ: return tj;
     case DYN_TUNE_MOMENTUM_TI:
     case DYN_TUNE_SCALAR_TI: return 16;
-    case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : 2;
+    case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : (c->pp.half_y ? 3 : 2);
     case DYN_TUNE_STAGE_STREAMS: return 2;
     default: return DYN_E_ARG;
     }

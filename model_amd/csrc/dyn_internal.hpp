@@ -72,6 +72,8 @@ struct PressurePlan {
     double *ffac = nullptr;            // Thomas multipliers F2 [kx][kk][ky] (FP64)
     double *zpiv = nullptr;            // Thomas pivot reciprocals [kx][kk][ky] (FP64)
     bool fused_y = false;              // j-transforms and the vertical solve in one LDS-resident kernel
+    bool half_y = false;               // ... or in two per kx on half slabs (even nj; one radix-2 split)
+    void *spec2 = nullptr;             // output of the half-slab j pass ([kx][j][kk] complex, T)
     size_t lds_max = 0;
 };
 

@@ -96,6 +96,37 @@ __device__ __forceinline__ void butterfly(int R, T *xr, T *xi, int sign) {
     }
 }
 
+// in-register DFT of length 8: two length-4 DFTs of the even and odd inputs, combined with
+// w^k = exp(sign i pi k / 4)
+template <typename T>
+__device__ __forceinline__ void butterfly8(T *xr, T *xi, int sign) {
+    T er[4] = {xr[0], xr[2], xr[4], xr[6]}, ei[4] = {xi[0], xi[2], xi[4], xi[6]};
+    T orr[4] = {xr[1], xr[3], xr[5], xr[7]}, oi[4] = {xi[1], xi[3], xi[5], xi[7]};
+    butterfly(4, er, ei, sign);
+    butterfly(4, orr, oi, sign);
+    const T h = T(0.70710678118654752440), s = T(sign);
+    // w^1 = h (1 + s i), w^2 = s i, w^3 = h (-1 + s i)
+    T t1r = h * (orr[1] - s * oi[1]), t1i = h * (oi[1] + s * orr[1]);
+    T t2r = -s * oi[2], t2i = s * orr[2];
+    T t3r = h * (-orr[3] - s * oi[3]), t3i = h * (-oi[3] + s * orr[3]);
+    xr[0] = er[0] + orr[0];
+    xi[0] = ei[0] + oi[0];
+    xr[4] = er[0] - orr[0];
+    xi[4] = ei[0] - oi[0];
+    xr[1] = er[1] + t1r;
+    xi[1] = ei[1] + t1i;
+    xr[5] = er[1] - t1r;
+    xi[5] = ei[1] - t1i;
+    xr[2] = er[2] + t2r;
+    xi[2] = ei[2] + t2i;
+    xr[6] = er[2] - t2r;
+    xi[6] = ei[2] - t2i;
+    xr[3] = er[3] + t3r;
+    xi[3] = ei[3] + t3i;
+    xr[7] = er[3] - t3r;
+    xi[7] = ei[3] - t3i;
+}
+
 // Stockham DFT of kc interleaved batches held in LDS as [n][kc] (re and im planes); ping-pongs
 // between buffer 0 and 1 and returns the index of the buffer holding the result.
 template <typename T>
@@ -398,7 +429,8 @@ __device__ __forceinline__ void fft_pass_pow2(T *re, T *im, int n, int Ns, int l
                 cmul(xr[t][r], xi[t][r], wr, wi);
             }
         }
-        butterfly(R, xr[t], xi[t], sign);
+        if constexpr (R == 8) butterfly8(xr[t], xi[t], sign);
+        else butterfly(R, xr[t], xi[t], sign);
         oidx[t] = ok ? (jd * Ns * R + jm) * sn + b * sb : -1;
     }
     __syncthreads();
@@ -413,6 +445,57 @@ __device__ __forceinline__ void fft_pass_pow2(T *re, T *im, int n, int Ns, int l
         }
     }
     __syncthreads();
+}
+
+// butterflies a thread holds in a radix-8 pass of a plan whose other passes hold MAXB (the radix-8 pass
+// has half the radix-4 count; a third of MAXB keeps its registers within what the radix-4 passes use)
+constexpr int pow2_maxb8(int maxb) { return maxb / 3 > 0 ? maxb / 3 : 1; }
+
+// the power-of-two plans of lds_fft_ip alone (radix-4 passes, with one radix-2 or radix-8 pass when
+// log2 n is odd)
+template <typename T, int MAXB>
+__device__ void lds_fft_pow2(T *re, T *im, const Radix &rad, int nbatch, int sn, int sb, const T *twl, int sign) {
+    const int n = rad.n;
+    int bq[MAXB], jq[MAXB];
+#pragma unroll
+    for (int t = 0; t < MAXB; t++) {
+        const int idx = threadIdx.x + t * (int)blockDim.x;
+        bq[t] = idx % nbatch;
+        jq[t] = idx / nbatch;
+    }
+    int Ns = 1, lg = 0;
+    for (int s = 0; s < rad.nr; s++) {
+        if (rad.r[s] == 4) fft_pass_pow2<T, 4, MAXB>(re, im, n, Ns, lg, sn, sb, bq, jq, twl, sign);
+        else if (rad.r[s] == 8) fft_pass_pow2<T, 8, pow2_maxb8(MAXB)>(re, im, n, Ns, lg, sn, sb, bq, jq, twl, sign);
+        else fft_pass_pow2<T, 2, MAXB>(re, im, n, Ns, lg, sn, sb, bq, jq, twl, sign);
+        Ns *= rad.r[s];
+        lg = 31 - __clz(Ns);
+    }
+}
+
+// power-of-two plan of the half-slab j pass: radix-4 passes, the first one radix 8 when log2 n is odd
+// (so no radix-2 pass, whose butterflies would be twice as many per thread)
+bool factorize(int n, Radix &rad);
+
+inline bool pow2_plan48(int n, Radix &rad, bool radix8) {
+    if (n < 2 || (n & (n - 1)) != 0) return false;
+    if (!radix8) return factorize(n, rad);  // 4, ..., 4, 2
+    rad.n = n;
+    rad.nr = 0;
+    int m = n;
+    if (__builtin_ctz(n) % 2 == 1) {
+        if (n == 2) {
+            rad.r[rad.nr++] = 2;
+            return true;
+        }
+        rad.r[rad.nr++] = 8;
+        m /= 8;
+    }
+    while (m > 1 && rad.nr < kMaxRad) {
+        rad.r[rad.nr++] = 4;
+        m /= 4;
+    }
+    return m == 1;
 }
 
 template <typename T, int MAXB, bool POW2 = false>
@@ -504,6 +587,12 @@ template <typename T>
 constexpr int ypass_threads() { return sizeof(T) == 4 ? 1024 : 512; }
 template <typename T>
 constexpr int ypass_maxb() { return sizeof(T) == 4 ? kMaxBY : kMaxBY64; }
+// the half-slab j pass (power-of-two plans, radix 4 and one radix-8 pass): FP32 1024 threads with up to
+// 4 butterflies each, FP64 512 threads with up to 6 (what fits each thread's registers)
+template <typename T>
+constexpr int half_threads() { return sizeof(T) == 4 ? 1024 : 512; }
+template <typename T>
+constexpr int half_maxb() { return sizeof(T) == 4 ? 4 : 6; }
 
 // Q1: divergence3d_periodic prologue + forward real DFT along i; block = (j, chunk of kc levels, kc even).
 // The periodic column UP(NX-3) is read as UP(3), the value periodic_uvw_boundaries puts there.
@@ -570,7 +659,8 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
 // Q3: backward real DFT along i -> PI (pressure.cpp:878-892) with the ghost levels and periodic halos
 template <typename T>
 __device__ __forceinline__ void irfft_x_out_block(const Geo &g, const T *__restrict__ spec, T *__restrict__ PI, const Radix &rad,
-                                                  const double *tw, int nk2, int kc, int j, int chunk, unsigned char *smem) {
+                                                  const double *tw, int nk2, int kc, int j, int chunk, unsigned char *smem,
+                                                  int jhalf = 0) {
     const int n = rad.n, P = kc / 2, half = n / 2;
     T *re = (T *)smem, *im = re + n * P, *twl = im + n * P;
     const int kk0 = chunk * kc;
@@ -583,13 +673,29 @@ __device__ __forceinline__ void irfft_x_out_block(const Geo &g, const T *__restr
         int ms = m <= half ? m : n - m;
         long long o = ((long long)ms * g.NY + j) * nk2;
         T ar = T(0), ai = T(0), br = T(0), bi = T(0);
-        if (ka < nk2) {
-            ar = spec[(o + ka) * 2];
-            ai = spec[(o + ka) * 2 + 1];
-        }
-        if (kb < nk2) {
-            br = spec[(o + kb) * 2];
-            bi = spec[(o + kb) * 2 + 1];
+        if (jhalf) {
+            // the half-slab j pass left row j of the full inverse as v0(j') + s v1(j'), j' = j mod jhalf,
+            // with v0 in row j' and v1 in row jhalf + j' (s = +1 on the first half of the rows, -1 on the second)
+            const int jp = j < jhalf ? j : j - jhalf;
+            const T sg = j < jhalf ? T(1) : T(-1);
+            const long long o0 = ((long long)ms * g.NY + jp) * nk2, o1 = ((long long)ms * g.NY + jhalf + jp) * nk2;
+            if (ka < nk2) {
+                ar = spec[(o0 + ka) * 2] + sg * spec[(o1 + ka) * 2];
+                ai = spec[(o0 + ka) * 2 + 1] + sg * spec[(o1 + ka) * 2 + 1];
+            }
+            if (kb < nk2) {
+                br = spec[(o0 + kb) * 2] + sg * spec[(o1 + kb) * 2];
+                bi = spec[(o0 + kb) * 2 + 1] + sg * spec[(o1 + kb) * 2 + 1];
+            }
+        } else {
+            if (ka < nk2) {
+                ar = spec[(o + ka) * 2];
+                ai = spec[(o + ka) * 2 + 1];
+            }
+            if (kb < nk2) {
+                br = spec[(o + kb) * 2];
+                bi = spec[(o + kb) * 2 + 1];
+            }
         }
         if (ms == 0 || ms == half) {  // self-conjugate wavenumbers: only the real part is kept
             ai = T(0);
@@ -647,7 +753,7 @@ struct ThomasArgs {
 // The sweeps below walk the levels in chunks of kTC and load the next chunk's table values (pivots or
 // multipliers, one column of the [kx][kk][ky] tables) while the current chunk is eliminated, so the
 // dependent recurrence does not wait on global-memory latency level by level.
-constexpr int kTC = 8;
+constexpr int kThomasChunk = 8;  // levels per chunk of the Thomas sweeps
 
 // Forward elimination and back substitution of one real sequence (the real or the imaginary part) of one
 // (kx,ky) column whose values sit at x[kk*xs]; FP64 arithmetic with the pivots and multipliers of the
@@ -658,7 +764,7 @@ constexpr int kTC = 8;
 // acoef entries and the next chunk's table entries (clamped indices, so every load is unconditional) are
 // all issued before the chain starts, and only the last partial chunk takes guarded steps.  Branchy
 // chunks made the compiler wait on every load as soon as it was issued, prefetch included.
-template <typename T>
+template <typename T, int kTC = kThomasChunk>
 __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, double *mean) {
     const int N = ta.nk - 2;
     const long long fs = ta.nj, col = (long long)kx * N * ta.nj + ky;
@@ -737,7 +843,7 @@ __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, 
 // (0,0) column p_N is replaced by 0.  The transpose runs the two bidiagonal solves transposed and in
 // reverse order: q_1 = x_1, q_k = x_k - F_{k-1} q_{k-1} (q_N = 0 for (0,0)), then b_N = z_N q_N,
 // b_k = z_k (q_k - a_{k+1} b_{k+1}), result b / S.  The singular last pivot of (0,0) is never used.
-template <typename T>
+template <typename T, int kTC = kThomasChunk>
 __device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs) {
     const int N = ta.nk - 2;
     const long long fs = ta.nj, col = (long long)kx * N * ta.nj + ky;
@@ -871,9 +977,71 @@ __global__ void __launch_bounds__(kThreadsX) k_div_rfft_x(Geo g, Cols c, const T
 
 template <typename T>
 __global__ void __launch_bounds__(kThreadsX) k_irfft_x_out(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
-                                                          const double *tw, int nk2, int kc) {
+                                                          const double *tw, int nk2, int kc, int half) {
     extern __shared__ unsigned char smem[];
-    irfft_x_out_block<T>(g, spec, PI, rad, tw, nk2, kc, blockIdx.x, blockIdx.y, smem);
+    irfft_x_out_block<T>(g, spec, PI, rad, tw, nk2, kc, blockIdx.x, blockIdx.y, smem, half);
+}
+
+// Q2 on half slabs (even nj, L = nj/2): the j transform's first radix-2 split (decimation in frequency)
+// is done while loading, so block (kx, h) holds only a_h(j') = (x(j') + (-1)^h x(j'+L)) W^(h j'),
+// W = exp(-2 pi i / nj), j' < L, whose length-L transform is X(2m + h).  It solves the columns
+// ky = 2m + h, transforms back, and stores v_h(j') = conj(W)^(h j') u_h(j') in rows h L + j' of `out`; the
+// inverse along i combines row j of the full inverse as v_0(j') + (-1)^q v_1(j'), j = j' + qL.  Both
+// blocks of a kx read every row of `in`, so they write to a second array.
+template <typename T, bool TR>
+__device__ __forceinline__ void ypass_half_block(const T *__restrict__ in, T *__restrict__ out, int nk2, const Radix &rh,
+                                                 const double *twy, const ThomasArgs &ta, int pad, int kx, int h, unsigned char *smem) {
+    const int L = rh.n, nj = 2 * L, P2 = nk2 + pad;
+    T *re = (T *)smem, *im = re + L * P2;
+    double *mean = (double *)(re + 2 * L * P2);
+    T *twl = (T *)(mean + 2 * nk2 + 1);
+    // the length-L table is every other entry of the length-nj one
+    for (int e = threadIdx.x; e < L; e += blockDim.x) {
+        twl[2 * e] = T(twy[4 * e]);
+        twl[2 * e + 1] = T(twy[4 * e + 1]);
+    }
+    const long long base = (long long)kx * nj * nk2 * 2, second = 2LL * L * nk2;
+    for (int idx = threadIdx.x; idx < L * nk2; idx += blockDim.x) {
+        const int jp = idx / nk2, kk = idx - jp * nk2;
+        const long long o = base + 2LL * idx;
+        T ar, ai;
+        if (h == 0) {
+            ar = in[o] + in[o + second];
+            ai = in[o + 1] + in[o + second + 1];
+        } else {
+            const T dr = in[o] - in[o + second], di = in[o + 1] - in[o + second + 1];
+            ar = dr;
+            ai = di;
+            cmul(ar, ai, T(twy[2 * jp]), T(twy[2 * jp + 1]));
+        }
+        re[jp * P2 + kk] = ar;
+        im[jp * P2 + kk] = ai;
+    }
+    __syncthreads();
+    lds_fft_pow2<T, half_maxb<T>()>(re, im, rh, nk2, P2, 1, twl, -1);
+    if ((int)threadIdx.x < 2 * L) {
+        const int part = (int)threadIdx.x >= L, m = (int)threadIdx.x - part * L;
+        T *x = (part ? im : re) + m * P2;
+        if (TR) thomas_part_t<T>(ta, kx, 2 * m + h, x, 1);
+        else thomas_part<T>(ta, kx, 2 * m + h, x, 1, mean + part);
+    }
+    __syncthreads();
+    lds_fft_pow2<T, half_maxb<T>()>(re, im, rh, nk2, P2, 1, twl, +1);
+    for (int idx = threadIdx.x; idx < L * nk2; idx += blockDim.x) {
+        const int jp = idx / nk2, kk = idx - jp * nk2;
+        T vr = re[jp * P2 + kk], vi = im[jp * P2 + kk];
+        if (h == 1) cmul(vr, vi, T(twy[2 * jp]), -T(twy[2 * jp + 1]));
+        const long long o = base + 2LL * ((long long)h * L * nk2 + idx);
+        out[o] = vr;
+        out[o + 1] = vi;
+    }
+}
+
+template <typename T, bool TR = false>
+__global__ void __launch_bounds__(half_threads<T>()) k_ypass_half(const T *__restrict__ in, T *__restrict__ out, int nk2, Radix rh,
+                                                                   const double *twy, ThomasArgs ta, int pad) {
+    extern __shared__ unsigned char smem[];
+    ypass_half_block<T, TR>(in, out, nk2, rh, twy, ta, pad, blockIdx.x >> 1, blockIdx.x & 1, smem);
 }
 
 template <typename T, bool TR = false>
@@ -1040,9 +1208,19 @@ int solve_t(Ctx *c, double step, bool halos) {
         DYN_LAUNCH_CHECK();
         const int pad = 1;
         const size_t shy = y_pass_lds(pp.nj, pp.nk2, sizeof(T));
-        if (fused_shape(c, DYN_TUNE_PRESSURE_JPASS) == 1) {
+        const int jpass = fused_shape(c, DYN_TUNE_PRESSURE_JPASS);
+        int half = 0;
+        if (jpass == 1) {
             Timed _t(c, "p2_ypass", pts);
             hipLaunchKernelGGL(k_ypass<T>, dim3(nkx), dim3(ypass_threads<T>()), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
+            DYN_LAUNCH_CHECK();
+        } else if (jpass == 3) {
+            Radix rh{};
+            pow2_plan48(pp.nj / 2, rh, sizeof(T) == 8);
+            half = pp.nj / 2;
+            Timed _t(c, "p2_ypass", pts);
+            hipLaunchKernelGGL((k_ypass_half<T>), dim3(2 * nkx), dim3(half_threads<T>()), y_pass_lds(half, pp.nk2, sizeof(T)), c->stream,
+                               (const T *)spec, (T *)pp.spec2, pp.nk2, rh, pp.twy, ta, pad);
             DYN_LAUNCH_CHECK();
         } else {
             int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
@@ -1068,7 +1246,8 @@ int solve_t(Ctx *c, double step, bool halos) {
         }
         {
             Timed _t(c, "p5_irfft_x_out", pts);
-            hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)spec, (T *)c->pi, rx, pp.twx, pp.nk2, kc);
+            hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)(half ? (T *)pp.spec2 : spec), (T *)c->pi,
+                               rx, pp.twx, pp.nk2, kc, half);
         }
         DYN_LAUNCH_CHECK();
         return DYN_OK;
@@ -1207,9 +1386,19 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     DYN_LAUNCH_CHECK();
     const int pad = 1;
     const size_t shy = y_pass_lds(pp.nj, pp.nk2, sizeof(T));
-    if (fused_shape(c, DYN_TUNE_PRESSURE_JPASS) == 1) {
+    const int jpass = fused_shape(c, DYN_TUNE_PRESSURE_JPASS);
+    int half = 0;
+    if (jpass == 1) {
         Timed _t(c, "adj_ypass", pts);
         hipLaunchKernelGGL((k_ypass<T, true>), dim3(nkx), dim3(ypass_threads<T>()), shy, c->stream, spec, pp.nk2, ry, pp.twy, ta, pad);
+        DYN_LAUNCH_CHECK();
+    } else if (jpass == 3) {
+        Radix rh{};
+        pow2_plan48(pp.nj / 2, rh, sizeof(T) == 8);
+        half = pp.nj / 2;
+        Timed _t(c, "adj_ypass", pts);
+        hipLaunchKernelGGL((k_ypass_half<T, true>), dim3(2 * nkx), dim3(half_threads<T>()), y_pass_lds(half, pp.nk2, sizeof(T)),
+                           c->stream, (const T *)spec, (T *)pp.spec2, pp.nk2, rh, pp.twy, ta, pad);
         DYN_LAUNCH_CHECK();
     } else {
         int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
@@ -1227,7 +1416,8 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     }
     {
         Timed _t(c, "adj_irfft_x", pts);
-        hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)spec, r2, rx, pp.twx, pp.nk2, kc);
+        hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)(half ? (T *)pp.spec2 : spec), r2, rx, pp.twx,
+                           pp.nk2, kc, half);
     }
     DYN_LAUNCH_CHECK();
     {
@@ -1279,6 +1469,31 @@ int pressure_init(Ctx *c) {
         DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
         DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
     }
+    // half slabs (power-of-two nj/2): what a slab of nj/2 rows needs, with one thread per real sequence
+    // of the solve
+    if (pp.nj % 2 == 0 && pp.nj >= 4 && ((pp.nj / 2) & (pp.nj / 2 - 1)) == 0) {
+        Radix rh{};
+        const bool ok = pow2_plan48(pp.nj / 2, rh, !f32);
+        const size_t needh = y_pass_lds(pp.nj / 2, pp.nk2, c->elt);
+        const int hthr = f32 ? half_threads<float>() : half_threads<double>();
+        const int hmaxb = f32 ? half_maxb<float>() : half_maxb<double>();
+        bool fits = ok && needh <= pp.lds_max && pp.nj <= hthr;
+        for (int st = 0; fits && st < rh.nr; st++) {  // butterflies a thread holds in each pass
+            const int items = rh.n / rh.r[st] * pp.nk2, per = (items + hthr - 1) / hthr;
+            fits = per <= (rh.r[st] == 8 ? pow2_maxb8(hmaxb) : hmaxb);
+        }
+        pp.half_y = fits;
+        if (pp.half_y) {
+            DYN_HIP(hipMalloc(&pp.spec2, sizeof(double) * 2 * (size_t)pp.ni * pp.nj * pp.nk2));
+            if (f32) {
+                DYN_HIP(hipFuncSetAttribute((const void *)k_ypass_half<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)needh));
+                DYN_HIP(hipFuncSetAttribute((const void *)k_ypass_half<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)needh));
+            } else {
+                DYN_HIP(hipFuncSetAttribute((const void *)k_ypass_half<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)needh));
+                DYN_HIP(hipFuncSetAttribute((const void *)k_ypass_half<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)needh));
+            }
+        }
+    }
     DYN_HIP(hipMalloc(&pp.ffac, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
     DYN_HIP(hipMalloc(&pp.zpiv, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
     DYN_HIP(hipMalloc(&pp.coskx, sizeof(double) * pp.ni));
@@ -1320,7 +1535,7 @@ int pressure_coefficients(Ctx *c) {
 
 int pressure_free(Ctx *c) {
     PressurePlan &pp = c->pp;
-    void *ptrs[] = {pp.twx, pp.twy, pp.spec, pp.ffac, pp.zpiv, pp.coskx, pp.cosly, pp.acoef, pp.ccoef};
+    void *ptrs[] = {pp.twx, pp.twy, pp.spec, pp.spec2, pp.ffac, pp.zpiv, pp.coskx, pp.cosly, pp.acoef, pp.ccoef};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     pp = PressurePlan{};

@@ -302,6 +302,26 @@ def test_stage_on_two_streams_bitwise(precision, moist, graph):
         assert np.array_equal(a.get_field(ids.ACCRAIN), b.get_field(ids.ACCRAIN))
 
 
+@pytest.mark.parametrize("jpass", [1, 2, 3], ids=["one-slab", "three-kernels", "half-slabs"])
+@pytest.mark.parametrize("graph", [False, True], ids=["per-operator", "graph"])
+def test_j_pass_forms_against_oracle(jpass, graph):
+    """The three forms of the pressure solve's j pass (one LDS-resident slab per kx, three kernels, two
+    half slabs per kx after a radix-2 split) all hold the FP64 plain form to the oracle at 1e-11."""
+    from model_amd import core
+    c = configs.small_case(nx=24, ny=64, nz=14, microphysics=1, amplitude=0.5)
+    o = Oracle(c)
+    g = DynCore(c)
+    g.tune_set(core.DYN_TUNE_PRESSURE_JPASS, jpass)
+    assert g.tune_get(core.DYN_TUNE_PRESSURE_JPASS) == jpass
+    o.step(8)
+    (g.step if graph else g.step_reference)(8)
+    g.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR):
+        e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
+        assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
+    assert rel_err(g.get_field(ids.PI, 0), o.get_field(ids.PI, 0)) <= 1e-11
+
+
 @pytest.mark.parametrize("graph", [False, True], ids=["per-operator", "graph"])
 def test_odd_transform_length_against_oracle(graph):
     """ni = 15 and nj = 18 take the full-complex transform path (odd i length) of the pressure solve."""
