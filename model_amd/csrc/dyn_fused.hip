@@ -1,7 +1,8 @@
 // Fused stage advection for gfx950: advect_uvw_velocity + advect_theta + advect_microphysics_cell
 // (+ the fall speed and the interior part of zero_moisture) in one pass over HBM.
 //
-// Mapping: one 64-lane wavefront per j-row, lane = level k (NZ <= 64); a block of TJ rows marches
+// Mapping: one 64-lane wavefront per j-row, lane = level k (NZ <= 64; two wavefronts for 66 <= NZ <= 128,
+// which hand the values crossing levels 63/64 over in LDS); a block of TJ rows marches
 // along i over a chunk of TI columns, as the reference's slab loop does (advection.cpp:313-325):
 //   * i-stencils come from per-thread register queues (own j,k; columns i-3..i+3) and the east-face
 //     flux of column i is carried as the west face of column i+1 (fluxes.cpp:891,1322,1504);
@@ -68,25 +69,29 @@ struct StageArgs {
 // LDS: U UB (planes i, i+1) and V VB (planes i-1, i) on rows j0-3 .. j0+TJ+2, W WB (planes i-1, i) on rows
 // j0-1 .. j0+TJ-1, each in a ring of three slots (plane p in slot p % 3); north faces of rows j0-1 .. j0+TJ-1.
 // Register queues hold U UB V VB of the own row at columns i-2 .. i+3.
-template <typename T, int TJ>
+template <typename T, int TJ, int KW = 1>
 struct MomLds {
+    static constexpr int KPW = KP * KW;
     static constexpr int R = TJ + 6;
     static constexpr int RW = TJ + 1;
-    static constexpr size_t uv = 3 * 4 * (size_t)R * KP;    // [slot][U UB V VB][row][k]
-    static constexpr size_t w = 3 * 2 * (size_t)RW * KP;    // [slot][W WB][row][k]
-    static constexpr size_t nf = 2 * (size_t)(TJ + 1) * KP; // [u v][row + 1][k]
-    static constexpr size_t bytes = (uv + w + nf) * sizeof(T);
+    static constexpr size_t uv = 3 * 4 * (size_t)R * KPW;    // [slot][U UB V VB][row][k]
+    static constexpr size_t w = 3 * 2 * (size_t)RW * KPW;    // [slot][W WB][row][k]
+    static constexpr size_t nf = 2 * (size_t)(TJ + 1) * KPW; // [u v][row + 1][k]
+    static constexpr size_t x = KW == 2 ? 3 * (size_t)TJ : 0;  // [row][bu bv bk] from level 63 to 64
+    static constexpr size_t bytes = (uv + w + nf + x) * sizeof(T);
 };
 
 // one tile (TJ rows x ti columns x all levels) of the momentum stage; every thread of the block calls it
-template <typename T, int TJ, bool MOIST>
+template <typename T, int TJ, bool MOIST, int KW = 1>
 __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, const StageArgs &a, T step, int t, unsigned char *smem_raw) {
-    using L = MomLds<T, TJ>;
+    using L = MomLds<T, TJ, KW>;
+    constexpr int KPW = L::KPW;
     constexpr int R = L::R, RW = L::RW;
     constexpr int HH = (6 + TJ - 1) / TJ, HW = (RW + TJ - 1) / TJ;  // halo / W plane rows per wavefront
     T *uvp = (T *)smem_raw;
     T *wpl = uvp + L::uv;
     T *nface = wpl + L::w;
+    T *xch = nface + L::nf;
 
     const int k = threadIdx.x, r = threadIdx.y;
     const int j0 = a.jl + (t % a.nbj) * TJ;
@@ -101,8 +106,8 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
     const unsigned rowoff = (unsigned)(jc * NZ + kc);
     const int RO = r + 3, WO = r + 1;  // own row in the U/V and W planes
 
-    auto UV = [&](int slot, int f, int row) -> T * { return uvp + ((slot * 4 + f) * R + row) * KP; };
-    auto WW = [&](int slot, int f, int row) -> T * { return wpl + ((slot * 2 + f) * RW + row) * KP; };
+    auto UV = [&](int slot, int f, int row) -> T * { return uvp + ((slot * 4 + f) * R + row) * KPW; };
+    auto WW = [&](int slot, int f, int row) -> T * { return wpl + ((slot * 2 + f) * RW + row) * KPW; };
     const void *qsrc[4] = {a.cur[0], a.bar[0], a.cur[1], a.bar[1]};
     T q[4][6], qs[4];  // U UB V VB of the own row at columns i-2 .. i+3 (i-3 .. i+2 in the prologue)
 
@@ -238,7 +243,7 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
         auto north_faces = [&](int RR, T *nf) {
             const T *U0 = UV(c0, 0, 0), *UB0 = UV(c0, 1, 0), *V0 = UV(c0, 2, 0), *VB0 = UV(c0, 3, 0);
             const T *Vm = UV(cm1, 2, 0), *VBm = UV(cm1, 3, 0);
-#define PL(ptr, row) ptr[(row) * KP + k]
+#define PL(ptr, row) ptr[(row) * KPW + k]
             {
                 T va = T(0.5) * (PL(V0, RR + 1) + PL(Vm, RR + 1));
                 T vba = T(0.5) * (PL(VB0, RR + 1) + PL(VBm, RR + 1));
@@ -257,13 +262,13 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
         };
         T nf[2];
         north_faces(RO, nf);
-        nface[(0 * (TJ + 1) + r + 1) * KP + k] = nf[0];
-        nface[(1 * (TJ + 1) + r + 1) * KP + k] = nf[1];
+        nface[(0 * (TJ + 1) + r + 1) * KPW + k] = nf[0];
+        nface[(1 * (TJ + 1) + r + 1) * KPW + k] = nf[1];
         if (r == 0) {
             T sf[2];
             north_faces(RO - 1, sf);
-            nface[(0 * (TJ + 1)) * KP + k] = sf[0];
-            nface[(1 * (TJ + 1)) * KP + k] = sf[1];
+            nface[(0 * (TJ + 1)) * KPW + k] = sf[0];
+            nface[(1 * (TJ + 1)) * KPW + k] = sf[1];
         }
         // top faces (u_top, v_top) of this level; level 0 has none
         T tf[2];
@@ -299,22 +304,33 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
         }
         if (k == 0) tf[0] = tf[1] = T(0);
         // bottom face of level k = top face of level k-1; buoyancy of level k-1 (advection.cpp:42-58)
-        const T bu = __shfl(tf[0], km1), bv = __shfl(tf[1], km1);
+        T bu = __shfl(tf[0], km1), bv = __shfl(tf[1], km1);
         T bk = cb[0] / T(c.tbv[kc]);
         if (MOIST) bk = cb[0] / T(c.tbv[kc]) + T(0.61) * cb[1] - cb[2] - cb[3];
-        const T bkm = __shfl(bk, km1);
+        T bkm = __shfl(bk, km1);
+        // two wavefronts per column: level 64 (the second one's first lane) takes level 63's values from LDS
+        if (KW == 2 && k == KP - 1) {
+            xch[3 * r] = tf[0];
+            xch[3 * r + 1] = tf[1];
+            xch[3 * r + 2] = bk;
+        }
         __syncthreads();
+        if (KW == 2 && k == KP) {
+            bu = xch[3 * r];
+            bv = xch[3 * r + 1];
+            bkm = xch[3 * r + 2];
+        }
         if (active) {
-            const T su0 = nface[(0 * (TJ + 1) + r) * KP + k], sv0 = nface[(1 * (TJ + 1) + r) * KP + k];
+            const T su0 = nface[(0 * (TJ + 1) + r) * KPW + k], sv0 = nface[(1 * (TJ + 1) + r) * KPW + k];
             {  // u (advection.cpp:73-99)
                 const T *V0 = UV(c0, 2, 0), *Vm = UV(cm1, 2, 0);
-                T other = T(0.25) * (V0[RO * KP + k] + V0[(RO + 1) * KP + k] + Vm[RO * KP + k] + Vm[(RO + 1) * KP + k]);
+                T other = T(0.25) * (V0[RO * KPW + k] + V0[(RO + 1) * KPW + k] + Vm[RO * KPW + k] + Vm[(RO + 1) * KPW + k]);
                 T tend = -dtx * (east[0] - west[0]) - dty * (nf[0] - su0) - DTZ * (rw1 * tf[0] - rw0 * bu) * odr + dt * fj * other;
                 st<T>(a.out[0], p, um + step * tend);
             }
             {  // v (advection.cpp:104-128)
                 const T *U0 = UV(c0, 0, 0), *Up = UV(cp1, 0, 0);
-                T other = T(0.25) * (U0[RO * KP + k] + Up[RO * KP + k] + U0[(RO - 1) * KP + k] + Up[(RO - 1) * KP + k]);
+                T other = T(0.25) * (U0[RO * KPW + k] + Up[RO * KPW + k] + U0[(RO - 1) * KPW + k] + Up[(RO - 1) * KPW + k]);
                 T tend = -dtx * (east[1] - west[1]) - dty * (nf[1] - sv0) - DTZ * (rw1 * tf[1] - rw0 * bv) * odr - dt * fj * other;
                 st<T>(a.out[1], p, vm + step * tend);
             }
@@ -341,23 +357,29 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
 // LDS: theta' thetabar (qv qbar qc qr) of plane i on rows j0-3 .. j0+TJ+2 in a ring of two slots, north
 // faces of rows j0-1 .. j0+TJ-1.  Register queues: the same fields of the own row at columns i-2 .. i+3.
 // Own-column inputs (u at i+1, v at j+1, w, the M values) are loaded per wavefront.
-template <typename T, int TJ, bool MOIST>
+template <typename T, int TJ, bool MOIST, int KW = 1>
 struct ScalLds {
+    static constexpr int KPW = KP * KW;
     static constexpr int R = TJ + 6;
     static constexpr int NS = MOIST ? 6 : 2;
     static constexpr int NFS = MOIST ? 4 : 1;
-    static constexpr size_t s = 2 * NS * (size_t)R * KP;          // [slot][field][row][k]
-    static constexpr size_t nf = NFS * (size_t)(TJ + 1) * KP;     // [face][row + 1][k]
-    static constexpr size_t bytes = (s + nf) * sizeof(T);
+    static constexpr size_t s = 2 * NS * (size_t)R * KPW;          // [slot][field][row][k]
+    static constexpr size_t nf = NFS * (size_t)(TJ + 1) * KPW;     // [face][row + 1][k]
+    // two wavefronts per column: [row][w wbar] from level 64 to 63, [row][top faces, rain face] from 63 to 64
+    static constexpr size_t xw = KW == 2 ? 2 * (size_t)TJ : 0;
+    static constexpr size_t xb = KW == 2 ? (NFS + 1) * (size_t)TJ : 0;
+    static constexpr size_t bytes = (s + nf + xw + xb) * sizeof(T);
 };
 
-template <typename T, int TJ, bool MOIST>
+template <typename T, int TJ, bool MOIST, int KW = 1>
 __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, const StageArgs &a, T step, int t, unsigned char *smem_raw) {
-    using L = ScalLds<T, TJ, MOIST>;
+    using L = ScalLds<T, TJ, MOIST, KW>;
+    constexpr int KPW = L::KPW;
     constexpr int R = L::R, NS = L::NS, NFS = L::NFS;
     constexpr int HH = (6 + TJ - 1) / TJ;  // halo plane rows per wavefront
     T *spl = (T *)smem_raw;
     T *nface = spl + L::s;
+    T *xw = nface + L::nf, *xb = xw + L::xw;
 
     const int k = threadIdx.x, r = threadIdx.y;
     const int j0 = a.jl + (t % a.nbj) * TJ;
@@ -373,7 +395,7 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
     const unsigned topoff = (unsigned)(jc * NZ + NZ - 1);
     const int RO = r + 3;
 
-    auto SP = [&](int slot, int f) -> T * { return spl + (slot * NS + f) * R * KP; };
+    auto SP = [&](int slot, int f) -> T * { return spl + (slot * NS + f) * R * KPW; };
     const void *src[6] = {a.cur[3], a.bar[3], a.cur[4], a.bar[4], a.cur[5], a.cur[6]};
     T q[NS][6], qs[NS];  // the own row at columns i-2 .. i+3 (i-3 .. i+2 in the prologue)
 
@@ -396,14 +418,14 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
     auto store_plane = [&](int i, int qd) {
         const int slot = i % 2;
 #pragma unroll
-        for (int f = 0; f < NS; f++) SP(slot, f)[RO * KP + k] = q[f][qd];
+        for (int f = 0; f < NS; f++) SP(slot, f)[RO * KPW + k] = q[f][qd];
 #pragma unroll
         for (int h = 0; h < HH; h++) {
             const int hh = r + h * TJ;
             if (hh < 6) {
                 const int rr = halo_row(hh);
 #pragma unroll
-                for (int f = 0; f < NS; f++) SP(slot, f)[rr * KP + k] = ss[h][f];
+                for (int f = 0; f < NS; f++) SP(slot, f)[rr * KPW + k] = ss[h][f];
             }
         }
     };
@@ -463,6 +485,14 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
     }
     Col cc{}, cs{};
     fetch_col(i0, cc);
+    // two wavefronts per column: level 63 reads w, wbar of level 64 from LDS
+    auto publish_w = [&]() {
+        if (KW == 2 && k == KP) {
+            xw[2 * r] = cc.w0;
+            xw[2 * r + 1] = cc.wb0;
+        }
+    };
+    publish_w();
     fetch_plane(i0);
     store_plane(i0, 2);
     __syncthreads();
@@ -488,7 +518,7 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         east_faces(cc.u1, cc.ub1, east);
         // north faces (s_north) of row RR with v, vbar at the face
         auto north_faces = [&](int RR, T V1, T VB1, T *nf) {
-#define PL(f, row) SP(c0, f)[(row) * KP + k]
+#define PL(f, row) SP(c0, f)[(row) * KPW + k]
             T sn = i5(PL(0, RR - 2), PL(0, RR - 1), PL(0, RR), PL(0, RR + 1), PL(0, RR + 2), PL(0, RR + 3), sgn(VB1 + V1));
             T bn = i5(PL(1, RR - 2), PL(1, RR - 1), PL(1, RR), PL(1, RR + 1), PL(1, RR + 2), PL(1, RR + 3), sgn(V1));
             nf[0] = V1 * bn + (VB1 + V1) * sn;
@@ -505,17 +535,21 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         T nf[NFS];
         north_faces(RO, cc.vn, cc.vbn, nf);
 #pragma unroll
-        for (int f = 0; f < NFS; f++) nface[(f * (TJ + 1) + r + 1) * KP + k] = nf[f];
+        for (int f = 0; f < NFS; f++) nface[(f * (TJ + 1) + r + 1) * KPW + k] = nf[f];
         if (r == 0) {
             T sf[NFS];
             north_faces(RO - 1, cc.vs, cc.vbs, sf);
 #pragma unroll
-            for (int f = 0; f < NFS; f++) nface[(f * (TJ + 1)) * KP + k] = sf[f];
+            for (int f = 0; f < NFS; f++) nface[(f * (TJ + 1)) * KPW + k] = sf[f];
         }
         // top faces (s_top) of this level from the own column of the plane; w, wbar at k+1 by lane exchange
-        const T W1 = __shfl(cc.w0, kp1), WB1 = __shfl(cc.wb0, kp1);
+        T W1 = __shfl(cc.w0, kp1), WB1 = __shfl(cc.wb0, kp1);
+        if (KW == 2 && k == KP - 1) {
+            W1 = xw[2 * r];
+            WB1 = xw[2 * r + 1];
+        }
         auto s_top = [&](int fs, int fb) -> T {
-            const T *S = SP(c0, fs) + RO * KP, *B = SP(c0, fb) + RO * KP;
+            const T *S = SP(c0, fs) + RO * KPW, *B = SP(c0, fb) + RO * KPW;
             T st_, bt;
             if (low) {
                 st_ = i2(S[kc], S[kp1]);
@@ -532,28 +566,35 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         if (MOIST) {
             tf[1] = s_top(2, 3);
             {
-                const T *S = SP(c0, 4) + RO * KP;
+                const T *S = SP(c0, 4) + RO * KPW;
                 T wb = WB1 + W1;
                 T tq = low ? i2(S[kc], S[kp1]) : i3(S[km1], S[kc], S[kp1], S[kp2], sgn(wb));
                 tf[2] = wb * tq;
             }
             // fall speed of this level (kessler.cpp:222-252); level 0 stores level 1's, NZ-1 keeps the array's
-            if (k >= 1 && k <= NZ - 2) {
-                T qr = q[5][2];
+            auto fall = [&](T qr, int kk) -> T {
+                T v = T(0);
                 if (qr > T(1.0e-12)) {
-                    T rk = T(c.rhou[k]);
+                    T rk = T(c.rhou[kk]);
                     T qrr = fmax(qr * T(0.001) * rk, T(0));
                     T vtden = sqrt(T(c.rhou[1]) / rk);
-                    vtk = T(36.34) * (pow(qrr, T(0.1364))) * vtden;
+                    v = T(36.34) * (pow(qrr, T(0.1364))) * vtden;
                 }
+                return v;
+            };
+            if (k >= 1 && k <= NZ - 2) {
+                vtk = fall(q[5][2], k);
             } else if (k == NZ - 1) {
                 vtk = cc.vtt;
             }
             // rain top face with the fall speed (fluxes.cpp:734, 780; 966-1030)
+            const T *QRr = SP(c0, 5) + RO * KPW;
             const T v1l = __shfl(vtk, 1);
             const T vte = k == 0 ? v1l : vtk;
-            const T vk1 = __shfl(vte, kp1);
-            const T *QRr = SP(c0, 5) + RO * KP;
+            T vk1 = __shfl(vte, kp1);
+            // two wavefronts per column: level 63 evaluates level 64's fall speed itself (NZ >= 66, so
+            // level 64 is an interior level), from the same plane value the other wavefront uses
+            if (KW == 2 && k == KP - 1) vk1 = fall(QRr[KP], KP);
             if (k == 0) {
                 T t0 = T(0.5) * (QRr[0] + QRr[1]);
                 qrt = -T(0.5) * (v1l + v1l) * t0;
@@ -572,13 +613,23 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         T bot[NFS];
 #pragma unroll
         for (int f = 0; f < NFS; f++) bot[f] = __shfl(tf[f], km1);
-        const T qrb = MOIST ? __shfl(qrt, km1) : T(0);
+        T qrb = MOIST ? __shfl(qrt, km1) : T(0);
+        if (KW == 2 && k == KP - 1) {
+#pragma unroll
+            for (int f = 0; f < NFS; f++) xb[(NFS + 1) * r + f] = tf[f];
+            xb[(NFS + 1) * r + NFS] = qrt;
+        }
         __syncthreads();
+        if (KW == 2 && k == KP) {
+#pragma unroll
+            for (int f = 0; f < NFS; f++) bot[f] = xb[(NFS + 1) * r + f];
+            if (MOIST) qrb = xb[(NFS + 1) * r + NFS];
+        }
         T oth = T(0), oqv = T(0), oqc = T(0), oqr = T(0);  // this level's final values (active levels)
         if (active) {
             T south[NFS];
 #pragma unroll
-            for (int f = 0; f < NFS; f++) south[f] = nface[(f * (TJ + 1) + r) * KP + k];
+            for (int f = 0; f < NFS; f++) south[f] = nface[(f * (TJ + 1) + r) * KPW + k];
             // Z_ADVECT_BASE (advection.cpp:16-18)
             auto zbase = [&](const double *base) -> T {
                 return (T(0.50) * (T(c.rhow[k + 1]) * W1 * (T(base[k + 1]) - T(base[k])) +
@@ -650,21 +701,22 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
                 q[f][5] = qs[f];
             }
             cc = cs;
+            publish_w();
         }
         __syncthreads();
     }
 }
 
-template <typename T, int TJ, bool MOIST>
-__global__ void __launch_bounds__(KP *TJ) k_adv_momentum(Geo g, Cols c, StageArgs a, T step) {
+template <typename T, int TJ, bool MOIST, int KW>
+__global__ void __launch_bounds__(KP *KW *TJ) k_adv_momentum(Geo g, Cols c, StageArgs a, T step) {
     extern __shared__ unsigned char smem_raw[];
-    adv_momentum_tile<T, TJ, MOIST>(g, c, a, step, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
+    adv_momentum_tile<T, TJ, MOIST, KW>(g, c, a, step, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
 }
 
-template <typename T, int TJ, bool MOIST>
-__global__ void __launch_bounds__(KP *TJ) k_adv_scalars(Geo g, Cols c, StageArgs a, T step) {
+template <typename T, int TJ, bool MOIST, int KW>
+__global__ void __launch_bounds__(KP *KW *TJ) k_adv_scalars(Geo g, Cols c, StageArgs a, T step) {
     extern __shared__ unsigned char smem_raw[];
-    adv_scalars_tile<T, TJ, MOIST>(g, c, a, step, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
+    adv_scalars_tile<T, TJ, MOIST, KW>(g, c, a, step, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
 }
 
 // zero_moisture outside the advected box (the fused kernel clips the points it writes)
@@ -686,28 +738,30 @@ __global__ void k_zero_moisture_shell(Geo g, Cols c, T *__restrict__ QVP, T *__r
     }
 }
 
-template <typename T, int TJ, bool MOIST>
+template <typename T, int TJ, bool MOIST, int KW>
 int momentum_t(Ctx *c, const StageArgs &a0, T step, int ti) {
     StageArgs a = a0;
     a.ti = ti;
     a.nbj = (a.jh - a.jl + TJ - 1) / TJ;
     const int nbi = (a.ih - a.il + ti - 1) / ti;
-    const size_t sh = MomLds<T, TJ>::bytes;
+    const size_t sh = MomLds<T, TJ, KW>::bytes;
     Timed _t(c, "adv_momentum", (double)(a.ih - a.il) * (a.jh - a.jl) * (c->geo.NZ - 2));
-    hipLaunchKernelGGL((k_adv_momentum<T, TJ, MOIST>), dim3(a.nbj * nbi), dim3(KP, TJ), sh, c->stream, c->geo, c->cols, a, step);
+    hipLaunchKernelGGL((k_adv_momentum<T, TJ, MOIST, KW>), dim3(a.nbj * nbi), dim3(KP * KW, TJ), sh, c->stream, c->geo, c->cols, a,
+                       step);
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
 
-template <typename T, int TJ, bool MOIST>
+template <typename T, int TJ, bool MOIST, int KW>
 int scalars_t(Ctx *c, const StageArgs &a0, T step, int ti) {
     StageArgs a = a0;
     a.ti = ti;
     a.nbj = (a.jh - a.jl + TJ - 1) / TJ;
     const int nbi = (a.ih - a.il + ti - 1) / ti;
-    const size_t sh = ScalLds<T, TJ, MOIST>::bytes;
+    const size_t sh = ScalLds<T, TJ, MOIST, KW>::bytes;
     Timed _t(c, "adv_scalars", (double)(a.ih - a.il) * (a.jh - a.jl) * (c->geo.NZ - 2));
-    hipLaunchKernelGGL((k_adv_scalars<T, TJ, MOIST>), dim3(a.nbj * nbi), dim3(KP, TJ), sh, c->stream, c->geo, c->cols, a, step);
+    hipLaunchKernelGGL((k_adv_scalars<T, TJ, MOIST, KW>), dim3(a.nbj * nbi), dim3(KP * KW, TJ), sh, c->stream, c->geo, c->cols, a,
+                       step);
     DYN_LAUNCH_CHECK();
     return DYN_OK;
 }
@@ -729,15 +783,24 @@ int stage_t(Ctx *c, double step, int il, int ih, int jl, int jh, int parts) {
     const int mtj = fused_shape(c, DYN_TUNE_MOMENTUM_TJ), mti = fused_shape(c, DYN_TUNE_MOMENTUM_TI);
     const int stj = fused_shape(c, DYN_TUNE_SCALAR_TJ), sti = fused_shape(c, DYN_TUNE_SCALAR_TI);
     int rc = DYN_OK;
+    if (fused_waves(c) == 2) {  // two wavefronts per column: 2 or 4 rows per block
+        if (parts & kStageMomentum) {
+            if (mtj == 2) rc = momentum_t<T, 2, MOIST, 2>(c, a, T(step), mti);
+            else rc = momentum_t<T, 4, MOIST, 2>(c, a, T(step), mti);
+        }
+        if (rc || !(parts & kStageScalars)) return rc;
+        if (stj == 2) return scalars_t<T, 2, MOIST, 2>(c, a, T(step), sti);
+        return scalars_t<T, 4, MOIST, 2>(c, a, T(step), sti);
+    }
     if (parts & kStageMomentum) {
-        if (mtj == 2) rc = momentum_t<T, 2, MOIST>(c, a, T(step), mti);
-        else if (mtj == 4) rc = momentum_t<T, 4, MOIST>(c, a, T(step), mti);
-        else rc = momentum_t<T, 8, MOIST>(c, a, T(step), mti);
+        if (mtj == 2) rc = momentum_t<T, 2, MOIST, 1>(c, a, T(step), mti);
+        else if (mtj == 4) rc = momentum_t<T, 4, MOIST, 1>(c, a, T(step), mti);
+        else rc = momentum_t<T, 8, MOIST, 1>(c, a, T(step), mti);
     }
     if (rc || !(parts & kStageScalars)) return rc;
-    if (stj == 2) return scalars_t<T, 2, MOIST>(c, a, T(step), sti);
-    if (stj == 4) return scalars_t<T, 4, MOIST>(c, a, T(step), sti);
-    return scalars_t<T, 8, MOIST>(c, a, T(step), sti);
+    if (stj == 2) return scalars_t<T, 2, MOIST, 1>(c, a, T(step), sti);
+    if (stj == 4) return scalars_t<T, 4, MOIST, 1>(c, a, T(step), sti);
+    return scalars_t<T, 8, MOIST, 1>(c, a, T(step), sti);
 }
 
 template <typename T>
@@ -1013,8 +1076,9 @@ int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw) {
     a.zh = c->g.nx - 1;
     a.zjl = 1;
     a.zjh = c->g.ny - 1;
-    const int rows = 4;
-    dim3 blk(KP, rows), grd((c->g.ny + rows - 1) / rows, c->g.nx - 6);
+    // one wavefront per column, two when NZ > 64; 256 threads per block
+    const int kw = c->g.nz > KP ? 2 : 1, rows = 4 / kw;
+    dim3 blk(KP * kw, rows), grd((c->g.ny + rows - 1) / rows, c->g.nx - 6);
     if (uvw) {
         Timed _t(c, "post_uvw", (double)c->npts);
         hipLaunchKernelGGL(k_post_uvw<T>, grd, blk, 0, c->stream, c->geo, c->cols, a, T(step), pgf);
@@ -1043,12 +1107,38 @@ int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw) {
 
 }  // namespace
 
-// one wavefront per column (NZ <= 64) and 32-bit byte offsets into every field
-bool fused_supported(const Ctx *c) { return c->g.nz <= KP && (double)c->npts * (double)c->elt < 4294967296.0; }
+// Wavefronts per column of the stage kernels: one up to NZ = 64; two for NZ in [66, 128], where the lid's
+// source levels 1 and NZ-2 sit in the wavefronts of the levels they feed (0 and NZ-1); 0 otherwise
+int fused_waves(const Ctx *c) {
+    const int nz = c->g.nz;
+    return nz <= KP ? 1 : (nz >= KP + 2 && nz <= 2 * KP ? 2 : 0);
+}
 
-// launch-shape knobs of the stage kernels: rows per block (instantiated widths) and march length
+// ... and 32-bit byte offsets into every field
+bool fused_supported(const Ctx *c) { return fused_waves(c) > 0 && (double)c->npts * (double)c->elt < 4294967296.0; }
+
+namespace {
+template <typename T, int KW>
+size_t stage_lds_t(bool momentum, bool moist, int tj) {
+    if (tj == 2) return momentum ? MomLds<T, 2, KW>::bytes : moist ? ScalLds<T, 2, true, KW>::bytes : ScalLds<T, 2, false, KW>::bytes;
+    if (tj == 4) return momentum ? MomLds<T, 4, KW>::bytes : moist ? ScalLds<T, 4, true, KW>::bytes : ScalLds<T, 4, false, KW>::bytes;
+    return momentum ? MomLds<T, 8, KW>::bytes : moist ? ScalLds<T, 8, true, KW>::bytes : ScalLds<T, 8, false, KW>::bytes;
+}
+// LDS bytes per block of a stage kernel with tj rows
+size_t stage_lds(const Ctx *c, bool momentum, int tj) {
+    const bool moist = c->g.microphysics != 0, two = fused_waves(c) == 2;
+    if (c->prec == DYN_FP32) return two ? stage_lds_t<float, 2>(momentum, moist, tj) : stage_lds_t<float, 1>(momentum, moist, tj);
+    return two ? stage_lds_t<double, 2>(momentum, moist, tj) : stage_lds_t<double, 1>(momentum, moist, tj);
+}
+}  // namespace
+
+// launch-shape knobs of the stage kernels: rows per block (instantiated widths: 2, 4, 8 with one wavefront
+// per column, 2, 4 with two; within the LDS) and march length
 bool fused_shape_ok(const Ctx *c, int param, int value) {
-    if (param == DYN_TUNE_MOMENTUM_TJ || param == DYN_TUNE_SCALAR_TJ) return value == 2 || value == 4 || value == 8;
+    if (param == DYN_TUNE_MOMENTUM_TJ || param == DYN_TUNE_SCALAR_TJ) {
+        const bool inst = value == 2 || value == 4 || (value == 8 && fused_waves(c) != 2);
+        return inst && stage_lds(c, param == DYN_TUNE_MOMENTUM_TJ, value) <= c->pp.lds_max;
+    }
     if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
     if (param == DYN_TUNE_PRESSURE_JPASS) return value == 2 || (value == 1 && c->pp.fused_y) || (value == 3 && c->pp.half_y);
     if (param == DYN_TUNE_STAGE_STREAMS) return value == 1 || value == 2;
@@ -1063,8 +1153,15 @@ int fused_shape(const Ctx *c, int param) {
     const int rows8 = ((c->g.ny - 6 + 7) / 8) * ((c->g.nx - 6 + 15) / 16);
     const int tj = f32 || rows8 >= 512 ? 8 : 4;
     switch (param) {
-    case DYN_TUNE_MOMENTUM_TJ: return tj;
-    case DYN_TUNE_SCALAR_TJ: return tj;
+    case DYN_TUNE_MOMENTUM_TJ:
+    case DYN_TUNE_SCALAR_TJ:
+        // two wavefronts per column: blocks of 512 threads (4 rows) where the LDS allows and the moist
+        // FP64 scalar kernel keeps its registers (at 4 rows it spills), else 2 rows
+        if (fused_waves(c) == 2) {
+            if (!f32 && param == DYN_TUNE_SCALAR_TJ && c->g.microphysics) return 2;
+            return fused_shape_ok(c, param, 4) ? 4 : 2;
+        }
+        return tj;
     case DYN_TUNE_MOMENTUM_TI:
     case DYN_TUNE_SCALAR_TI: return 16;
     case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : (c->pp.half_y ? 3 : 2);
@@ -1073,32 +1170,36 @@ int fused_shape(const Ctx *c, int param) {
     }
 }
 
-template <typename T, int TJ>
-int set_lds_limits() {
-    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, TJ, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)MomLds<T, TJ>::bytes));
-    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, TJ, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)MomLds<T, TJ>::bytes));
-    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, TJ, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)ScalLds<T, TJ, true>::bytes));
-    DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, TJ, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)ScalLds<T, TJ, false>::bytes));
+// shapes whose LDS exceeds the device's are never launched (fused_shape_ok), so their limit stays unset
+template <typename T, int TJ, int KW>
+int set_lds_limits(const Ctx *c) {
+    const size_t mom = MomLds<T, TJ, KW>::bytes, sm = ScalLds<T, TJ, true, KW>::bytes, sd = ScalLds<T, TJ, false, KW>::bytes;
+    if (mom <= c->pp.lds_max) {
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, TJ, true, KW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mom));
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, TJ, false, KW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mom));
+    }
+    if (sm <= c->pp.lds_max)
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, TJ, true, KW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    if (sd <= c->pp.lds_max)
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, TJ, false, KW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sd));
     return DYN_OK;
 }
 
-// the stage kernels may use more than 64 KiB of LDS: raise the per-launch limit once, outside any capture
-int fused_prepare(Ctx *c) {
-    if (c->prec == DYN_FP32) {
-        int rc = set_lds_limits<float, 2>();
-        if (!rc) rc = set_lds_limits<float, 4>();
-        if (!rc) rc = set_lds_limits<float, 8>();
+template <typename T>
+int set_lds_limits_all(const Ctx *c) {
+    if (fused_waves(c) == 2) {
+        int rc = set_lds_limits<T, 2, 2>(c);
+        if (!rc) rc = set_lds_limits<T, 4, 2>(c);
         return rc;
     }
-    int rc = set_lds_limits<double, 2>();
-    if (!rc) rc = set_lds_limits<double, 4>();
-    if (!rc) rc = set_lds_limits<double, 8>();
+    int rc = set_lds_limits<T, 2, 1>(c);
+    if (!rc) rc = set_lds_limits<T, 4, 1>(c);
+    if (!rc) rc = set_lds_limits<T, 8, 1>(c);
     return rc;
 }
+
+// the stage kernels may use more than 64 KiB of LDS: raise the per-launch limit once, outside any capture
+int fused_prepare(Ctx *c) { return c->prec == DYN_FP32 ? set_lds_limits_all<float>(c) : set_lds_limits_all<double>(c); }
 
 // advect_uvw_velocity + advect_theta (+ advect_microphysics_cell and the interior zero_moisture)
 // over the box [il,ih) x [jl,jh)

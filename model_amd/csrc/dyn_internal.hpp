@@ -204,6 +204,7 @@ void adjoint_release(Ctx *c);
 typedef int (*StageHook)(Ctx *c, int stage, void *arg);
 int step_with_hook(Ctx *c, StageHook hook, void *arg);
 bool fused_supported(const Ctx *c);
+int fused_waves(const Ctx *c);
 // ensembles: batched launches over a device table of members (dyn_fused.hip, dyn_pressure.hip)
 int ens_prepare(bool fp32);
 int launch_ens_stage(Ctx *c0, const EnsMember *d, int nmem, const EnsRoles &ro, double step, const int *shape, int tot_mom, int tot_scal);

@@ -14,7 +14,7 @@ import pytest
 
 from helpers import rel_err
 from model_amd import configs, ids
-from model_amd.core import DYN_FP32, DynCore, REFAPI_PATH
+from model_amd.core import DYN_FP32, DynCore, DynError, REFAPI_PATH
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -180,8 +180,9 @@ def test_c2_full_size_properties():
 
 
 def test_c5_full_size_properties():
-    """At the 512x512x80 moist size in FP32 (NZ above one wavefront, so the per-operator kernels under
-    graph replay): a few steps stay finite and rain and cloud stay non-negative; the run is timed."""
+    """At the 512x512x80 moist size in FP32 (NZ above one wavefront: the fused stage kernels with two
+    wavefronts per column, under graph replay): a few steps stay finite and rain and cloud stay
+    non-negative; the run is timed."""
     import time
     c = configs.tropical_vortex()
     g = DynCore(c, precision=DYN_FP32)
@@ -278,6 +279,81 @@ def test_fused_launch_shapes_bitwise(precision, shape):
     for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR, ids.VT):
         role = CUR if f in ids.PROGNOSTIC else 0
         assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), ids.FIELD_NAMES[f]
+
+
+@pytest.mark.parametrize("nz", [66, 80, 128])
+@pytest.mark.parametrize("precision,moist", [(DYN_FP32, 1), (DYN_FP32, 0), (0, 1), (0, 0)],
+                         ids=["fp32-moist", "fp32-dry", "fp64-moist", "fp64-dry"])
+def test_tall_columns_fused_matches_per_operator_bitwise(precision, moist, nz):
+    """66 <= NZ <= 128: the fused stage kernels run two wavefronts per column and hand the values that
+    cross levels 63/64 over in LDS; after several steps every buffer agrees with the per-operator kernels
+    to the last bit (the 64/65 pair of levels included)."""
+    c = configs.supercell(nx=24, ny=20, nz=nz) if moist else configs.warm_bubble(nx=24, ny=20, nz=nz)
+    a, b = DynCore(c, precision=precision), DynCore(c, precision=precision)
+    a.step_reference(4)
+    b.step_fused(4)
+    a.sync()
+    b.sync()
+    fields = (ids.U, ids.V, ids.W, ids.TH) + ((ids.QV, ids.QC, ids.QR) if moist else ())
+    for f in fields:
+        for role in (ids.ROLE_M, CUR, P):
+            x, y = a.get_field(f, role), b.get_field(f, role)
+            assert np.array_equal(x, y), (ids.FIELD_NAMES[f], role, np.abs(x - y).max())
+    if moist:
+        assert np.array_equal(a.get_field(ids.VT, 0), b.get_field(ids.VT, 0))
+        assert np.array_equal(a.get_field(ids.ACCRAIN), b.get_field(ids.ACCRAIN))
+
+
+@pytest.mark.parametrize("precision,shape", [(DYN_FP32, (2, 8, 2, 32)), (DYN_FP32, (4, 5, 4, 7)), (0, (2, 16, 4, 9))],
+                         ids=["fp32-2rows", "fp32-4rows", "fp64-dry-4rows"])
+def test_tall_column_launch_shapes_bitwise(precision, shape):
+    """Two wavefronts per column (NZ = 90): 2 or 4 rows per block and any march length give the same
+    bits; 8 rows per block is not offered."""
+    from model_amd import core
+    c = configs.warm_bubble(nx=24, ny=20, nz=90)
+    a, b = DynCore(c, precision=precision), DynCore(c, precision=precision)
+    knobs = (core.DYN_TUNE_MOMENTUM_TJ, core.DYN_TUNE_MOMENTUM_TI, core.DYN_TUNE_SCALAR_TJ, core.DYN_TUNE_SCALAR_TI)
+    for knob, v in zip(knobs, shape):
+        b.tune_set(knob, v)
+        assert b.tune_get(knob) == v
+    with pytest.raises(DynError):
+        b.tune_set(core.DYN_TUNE_SCALAR_TJ, 8)
+    a.step_fused(3)
+    b.step_fused(3)
+    a.sync()
+    b.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH):
+        assert np.array_equal(a.get_field(f, CUR), b.get_field(f, CUR)), ids.FIELD_NAMES[f]
+
+
+def test_tall_columns_fp64_against_oracle():
+    """The moist supercell at NZ = 70 (two wavefronts per column), graph replay, FP64: 1e-11 to the
+    oracle after 10 steps."""
+    c = configs.supercell(nx=32, ny=24, nz=70)
+    o = Oracle(c)
+    o.step(10)
+    g = DynCore(c)
+    g.step(10)
+    g.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR):
+        e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
+        assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
+
+
+def test_nz65_takes_the_per_operator_step():
+    """NZ = 65 puts the lid source level NZ-2 = 63 in the other wavefront of level NZ-1 = 64: the fused
+    step refuses it and the graph replays the per-operator kernels, still at 1e-11 to the oracle."""
+    c = configs.warm_bubble(nx=24, ny=20, nz=65)
+    o = Oracle(c)
+    o.step(3)
+    g = DynCore(c)
+    with pytest.raises(DynError):
+        g.step_fused(1)
+    g.step(3)
+    g.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH):
+        e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
+        assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
 
 
 @pytest.mark.parametrize("precision,moist", [(DYN_FP32, 1), (DYN_FP32, 0), (0, 1)], ids=["fp32-moist", "fp32-dry", "fp64-moist"])
