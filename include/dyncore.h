@@ -102,11 +102,12 @@ int dyn_micro_advance_outer(dyn_ctx *ctx);
 /* ---- whole steps ------------------------------------------------------------------------- */
 /* s_integrate_rk3 (driver_serial.cpp:178-286) as the sequence of per-operator calls above */
 int dyn_step_reference(dyn_ctx *ctx, int nsteps);
-/* the step replayed from a captured hipGraph: when NZ <= 64 the stage advection runs as the fused
- * momentum and scalar kernels with fused boundary epilogues (same values as the per-operator kernels,
- * bit for bit, in either precision); otherwise the per-operator sequence is replayed */
+/* the step replayed from a captured hipGraph: when NZ <= 64 or 66 <= NZ <= 128 the stage advection runs
+ * as the fused momentum and scalar kernels with fused boundary epilogues (same values as the per-operator
+ * kernels, bit for bit, in either precision); otherwise the per-operator sequence is replayed */
 int dyn_step(dyn_ctx *ctx, int nsteps);
-/* the fused-stage step launched eagerly (either precision; needs NZ <= 64 and fields below 4 GiB) */
+/* the fused-stage step launched eagerly (either precision; needs NZ <= 64 or 66 <= NZ <= 128, and
+ * fields below 4 GiB) */
 int dyn_step_fused(dyn_ctx *ctx, int nsteps);
 
 /* ---- measurement ------------------------------------------------------------------------- */

@@ -1163,7 +1163,7 @@ int fused_shape(const Ctx *c, int param) {
         }
         return tj;
     case DYN_TUNE_MOMENTUM_TI:
-    case DYN_TUNE_SCALAR_TI: return 16;
+    case DYN_TUNE_SCALAR_TI: return fused_waves(c) == 2 ? 32 : 16;  // 512x512x80: 32 measured 3 % faster
     case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : (c->pp.half_y ? 3 : 2);
     case DYN_TUNE_STAGE_STREAMS: return 2;
     default: return DYN_E_ARG;
