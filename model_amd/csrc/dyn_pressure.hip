@@ -580,6 +580,61 @@ __device__ __forceinline__ void load_twiddles(T *twl, const double *tw, int n) {
     for (int e = threadIdx.x; e < 2 * n; e += blockDim.x) twl[e] = T(tw[e]);
 }
 
+// j transforms of the three-kernel j pass for power-of-two nj: kc levels of one kx in LDS as [ky][level]
+// (one padding level), transformed in place by the straight-line radix-8/4 passes with the twiddles in
+// LDS; the same layout in HBM as k_fft_y
+// 512 threads with two butterflies each keep the registers low (two blocks per CU) and still take 8
+// FP32 or 8 FP64 levels per block, whole 64- or 128-byte runs of the spectrum per ky
+constexpr int kFftYThreads = 512, kFftYMaxB = 2;
+constexpr size_t kFftYLds = 96 * 1024;
+template <typename T, int MAXB>
+__global__ void __launch_bounds__(kFftYThreads) k_fft_y_pow2(T *__restrict__ spec, int nj, int nk2, Radix rad, const double *tw, int kc,
+                                                         int sign) {
+    extern __shared__ unsigned char smem[];
+    const int n = rad.n, P2 = kc + 1;
+    T *re = (T *)smem, *im = re + n * P2, *twl = im + n * P2;
+    load_twiddles(twl, tw, n);
+    const int kx = blockIdx.x, kk0 = blockIdx.y * kc;
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        const int b = idx % kc, ky = idx / kc, kk = kk0 + b;
+        T vr = T(0), vi = T(0);
+        if (kk < nk2) {
+            const long long o = (((long long)kx * nj + ky) * nk2 + kk) * 2;
+            vr = spec[o];
+            vi = spec[o + 1];
+        }
+        re[ky * P2 + b] = vr;
+        im[ky * P2 + b] = vi;
+    }
+    __syncthreads();
+    lds_fft_pow2<T, MAXB>(re, im, rad, kc, P2, 1, twl, sign);
+    for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+        const int b = idx % kc, ky = idx / kc, kk = kk0 + b;
+        if (kk < nk2) {
+            const long long o = (((long long)kx * nj + ky) * nk2 + kk) * 2;
+            spec[o] = re[ky * P2 + b];
+            spec[o + 1] = im[ky * P2 + b];
+        }
+    }
+}
+inline size_t fft_y_pow2_lds(int n, int kc, size_t elt) { return 2 * (size_t)n * (kc + 1) * elt + 2 * (size_t)n * elt; }
+// levels per block of k_fft_y_pow2 for the plan `rad` (the most that fit the LDS budget and the
+// butterflies a thread holds in every pass), 0 when none do
+template <typename T>
+int fft_y_pow2_chunk(const Radix &rad, int nk2) {
+    const int n = rad.n, maxb = kFftYMaxB;
+    for (int kc = nk2 < 32 ? nk2 : 32; kc >= 1; kc--) {
+        if (fft_y_pow2_lds(n, kc, sizeof(T)) > kFftYLds) continue;
+        bool ok = true;
+        for (int st = 0; ok && st < rad.nr; st++) {
+            const int items = n / rad.r[st] * kc, per = (items + kFftYThreads - 1) / kFftYThreads;
+            ok = per <= (rad.r[st] == 8 ? pow2_maxb8(maxb) : maxb);
+        }
+        if (ok) return kc;
+    }
+    return 0;
+}
+
 constexpr int kMaxBX = 4;   // butterflies per thread in the x passes (register-resident)
 constexpr int kMaxBY = 4;   // and in the fused y pass (FP32, 1024 threads)
 constexpr int kMaxBY64 = 6; // the fused y pass in FP64 runs 512 threads with up to 6 butterflies each
@@ -1133,6 +1188,25 @@ bool factorize(int n, Radix &rad) {
     return m == 1;
 }
 
+int level_chunk(int n, size_t elt, int nk2);
+
+// one j transform of the three-kernel j pass over nrows kx rows: the power-of-two kernel when nj is a
+// power of two and its plan fits, else the mixed-radix one
+template <typename T>
+void launch_fft_y(Ctx *c, T *spec, int nrows, const Radix &ry, int sign) {
+    const PressurePlan &pp = c->pp;
+    Radix rp{};
+    const int kcp = pow2_plan48(pp.nj, rp, true) ? fft_y_pow2_chunk<T>(rp, pp.nk2) : 0;
+    if (kcp > 0) {
+        hipLaunchKernelGGL((k_fft_y_pow2<T, kFftYMaxB>), dim3(nrows, (pp.nk2 + kcp - 1) / kcp), dim3(kFftYThreads),
+                           fft_y_pow2_lds(pp.nj, kcp, sizeof(T)), c->stream, spec, pp.nj, pp.nk2, rp, pp.twy, kcp, sign);
+        return;
+    }
+    const int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
+    hipLaunchKernelGGL(k_fft_y<T>, dim3(nrows, (pp.nk2 + kcy - 1) / kcy), dim3(kThreads), 4 * (size_t)pp.nj * kcy * sizeof(T), c->stream,
+                       spec, pp.nj, pp.nk2, ry, pp.twy, kcy, sign);
+}
+
 int level_chunk(int n, size_t elt, int nk2) {
     // two ping-pong buffers of n*kc complex values must fit the LDS budget
     int kc = (int)(kLdsBudget / (4 * (size_t)n * elt));
@@ -1223,12 +1297,9 @@ int solve_t(Ctx *c, double step, bool halos) {
                                (const T *)spec, (T *)pp.spec2, pp.nk2, rh, pp.twy, ta, pad);
             DYN_LAUNCH_CHECK();
         } else {
-            int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
-            size_t sh2 = 4 * (size_t)pp.nj * kcy * sizeof(T);
-            dim3 gy(nkx, (pp.nk2 + kcy - 1) / kcy);
             {
                 Timed _t(c, "p2_fft_y", pts);
-                hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
+                launch_fft_y<T>(c, spec, nkx, ry, -1);
             }
             DYN_LAUNCH_CHECK();
             const int tg = 64;
@@ -1240,7 +1311,7 @@ int solve_t(Ctx *c, double step, bool halos) {
             DYN_LAUNCH_CHECK();
             {
                 Timed _t(c, "p4_ifft_y", pts);
-                hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
+                launch_fft_y<T>(c, spec, nkx, ry, +1);
             }
             DYN_LAUNCH_CHECK();
         }
@@ -1401,17 +1472,14 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
                            c->stream, (const T *)spec, (T *)pp.spec2, pp.nk2, rh, pp.twy, ta, pad);
         DYN_LAUNCH_CHECK();
     } else {
-        int kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
-        size_t sh2 = 4 * (size_t)pp.nj * kcy * sizeof(T);
-        dim3 gy(nkx, (pp.nk2 + kcy - 1) / kcy);
         Timed _t(c, "adj_ypass", pts);
-        hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
+        launch_fft_y<T>(c, spec, nkx, ry, -1);
         DYN_LAUNCH_CHECK();
         const int tg = 64;
         size_t sht = ((tg * (2 * (size_t)pp.nk2 + 1) * sizeof(T) + 7) / 8) * 8 + 16 * (size_t)pp.nk2;
         hipLaunchKernelGGL((k_thomas_staged<T, true>), dim3(nkx, (pp.nj + tg - 1) / tg), dim3(2 * tg), sht, c->stream, spec, pp.nk2, ta);
         DYN_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), sh2, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, +1);
+        launch_fft_y<T>(c, spec, nkx, ry, +1);
         DYN_LAUNCH_CHECK();
     }
     {
@@ -1494,6 +1562,9 @@ int pressure_init(Ctx *c) {
             }
         }
     }
+    // the power-of-two j transforms of the three-kernel j pass may take more than 64 KiB of LDS
+    if (f32) DYN_HIP(hipFuncSetAttribute((const void *)k_fft_y_pow2<float, kFftYMaxB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFftYLds));
+    else DYN_HIP(hipFuncSetAttribute((const void *)k_fft_y_pow2<double, kFftYMaxB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFftYLds));
     DYN_HIP(hipMalloc(&pp.ffac, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
     DYN_HIP(hipMalloc(&pp.zpiv, sizeof(double) * (size_t)pp.ni * pp.nj * pp.nk2));
     DYN_HIP(hipMalloc(&pp.coskx, sizeof(double) * pp.ni));
