@@ -275,7 +275,7 @@ def main():
     }
     if not args.no_extras:
         log("bench: ensemble (32 members)")
-        out["ensemble"] = ensemble_bench(max(10, args.steps // 5), 3)
+        out["ensemble"] = ensemble_bench(max(10, args.steps // 5), 6)
         log(f"bench: ensemble {out['ensemble']['value']:.4g} gridpoint-steps/s, "
             f"{out['ensemble']['batched_over_one_by_one']}x the members one by one")
         log("bench: adjoint (192x192x40, 360 steps)")

@@ -689,8 +689,11 @@ int backward_t(Ctx *c, const double *const *seed, double *const *grad) {
             if (rc) return rc;
             for (int n = n0; n < n1; n++) {
                 int local = n - n0;
-                rc = step_with_hook(c, record_hook, &local);
-                if (rc) return rc;
+                rc = step_with_hook(c, record_hook, &local, n < n1 - 1);
+                if (rc) {
+                    unshare_roles(c);
+                    return rc;
+                }
             }
             s->rec_seg = seg;
         }
@@ -788,8 +791,14 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
             if (rc) return rc;
         }
         int local = n - last0;
-        rc = n >= last0 ? step_with_hook(c, record_hook, &local) : step_with_hook(c, nullptr, nullptr);
-        if (rc) return rc;
+        // M and current share a buffer between the steps (the snapshots copy each role's values, the
+        // same whether or not they share); the last step separates them
+        const bool share = n < nsteps - 1;
+        rc = n >= last0 ? step_with_hook(c, record_hook, &local, share) : step_with_hook(c, nullptr, nullptr, share);
+        if (rc) {
+            unshare_roles(c);
+            return rc;
+        }
     }
     s->rec_seg = nseg - 1;
     rc = copy_state(c, &s->snap[(size_t)nseg * 12], nullptr, true);

@@ -54,7 +54,14 @@ void drop_graph(Ctx *c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
+    for (auto &sg : c->graphs) {
+        (void)hipGraphExecDestroy(sg.exec);
+        (void)hipGraphDestroy(sg.graph);
+    }
+    c->graphs.clear();
 }
+
+
 
 __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long long n) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -82,6 +89,27 @@ __global__ void __launch_bounds__(256) k_copy_fields(FieldCopies fc, long long w
 }
 __global__ void k_clear_flag(int *f) { *f = 0; }
 
+// The frame of a field: whole columns with i < 6, i >= NX-6, j < 6 or j >= NY-6 (NX, NY >= 12).  The
+// twelve outer i-slabs are contiguous runs of NY*NZ points, the rest two runs of 6*NZ per i.
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy_frame(FieldCopies fc, int NX, int NY, int NZ) {
+    const T *s = (const T *)fc.src[blockIdx.y];
+    T *d = (T *)fc.dst[blockIdx.y];
+    const long long slab = (long long)NY * NZ, na = 12 * slab, run = 12LL * NZ, nb = (long long)(NX - 12) * run;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < na + nb; e += (long long)gridDim.x * blockDim.x) {
+        long long p;
+        if (e < na) {
+            const int sl = (int)(e / slab);
+            p = (long long)(sl < 6 ? sl : NX - 12 + sl) * slab + e % slab;
+        } else {
+            const long long q = e - na, r = q % run;
+            const int i = 6 + (int)(q / run), jj = (int)(r / NZ);
+            p = ((long long)i * NY + (jj < 6 ? jj : NY - 12 + jj)) * NZ + r % NZ;
+        }
+        d[p] = s[p];
+    }
+}
+
 int device_copy(Ctx *c, void *dst, const void *src) {
     long long n = (long long)c->npts;
     unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
@@ -105,6 +133,38 @@ int device_copy_fields(Ctx *c, int f0, int f1) {
     Timed _t(c, "advance_copy", (double)c->npts * nf);
     hipLaunchKernelGGL(k_copy_fields, dim3(blocks, nf), dim3(256), 0, c->stream, fc, words);
     DYN_LAUNCH_CHECK();
+    return DYN_OK;
+}
+
+// Stage 2 of a step that starts with M and current sharing a buffer writes the spare buffer, which then
+// holds an older state than the reference's buffer (the step's starting state).  Stage 2 reads some
+// points of that buffer before it writes them: w's levels 0, 1 and NZ-1 everywhere (the divergence
+// reads w there) and, of every field, points of the halo and of the columns next to it (the
+// divergence's halo columns; the periodic copy carries levels 0 and NZ-1 of the source columns before
+// the lid).  The reference's step sequence, run on the CPU with the rest of that buffer poisoned,
+// gives the same bits as without the poison, so copying w whole and the six-column frame of the other
+// fields from the starting state gives every value the reference has.
+int device_copy_shell(Ctx *c, int f0, int f1) {
+    const int NX = c->g.nx, NY = c->g.ny, NZ = c->g.nz;
+    const long long words = (long long)(c->npts * c->elt / 4);
+    const unsigned wblocks = (unsigned)std::max<long long>(1, std::min<long long>((words / 4 + 255) / 256, 2048));
+    Timed _t(c, "shell_copy", (double)c->npts);
+    FieldCopies whole{}, frame{};
+    int nw = 0, nf = 0;
+    for (int f = f0; f <= f1; f++) {
+        const bool all = f == DYN_W || NX < 12 || NY < 12;  // narrow grids: no interior beyond the frame
+        (all ? whole.src[nw] : frame.src[nf]) = c->prog[f][DYN_ROLE_M];
+        (all ? whole.dst[nw++] : frame.dst[nf++]) = c->prog[f][DYN_ROLE_P];
+    }
+    hipLaunchKernelGGL(k_copy_fields, dim3(wblocks, nw), dim3(256), 0, c->stream, whole, words);
+    DYN_LAUNCH_CHECK();
+    if (nf > 0) {
+        const long long n = 12LL * NY * NZ + (long long)(NX - 12) * 12 * NZ;
+        const dim3 grid((unsigned)std::min<long long>((n + 255) / 256, 1024), nf);
+        if (c->prec == DYN_FP32) hipLaunchKernelGGL(k_copy_frame<float>, grid, dim3(256), 0, c->stream, frame, NX, NY, NZ);
+        else hipLaunchKernelGGL(k_copy_frame<double>, grid, dim3(256), 0, c->stream, frame, NX, NY, NZ);
+        DYN_LAUNCH_CHECK();
+    }
     return DYN_OK;
 }
 
@@ -293,8 +353,8 @@ int dyn_destroy(dyn_ctx *ctx) {
         (void)hipEventDestroy(r.b);
     }
     for (auto e : c->pool) (void)hipEventDestroy(e);
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    drop_graph(c);
+    unshare_roles(c);
     adjoint_release(c);
     pressure_free(c);
     for (auto &f : c->prog)
@@ -623,11 +683,19 @@ static int reference_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nu
 // advect_microphysics_cell and zero_moisture's interior in one pass).  advect_theta and the moisture
 // advection read only the current-stage fields, which the pressure solve and the gradient force never
 // write, so they move ahead of solve_pressure without changing any value.
-static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullptr) {
+//
+// share_end: end the step with the M and current roles on one buffer instead of copying M <- P (the
+// next step must then be a fused step too, and the call's last step must not share).  A step that
+// starts shared reads M and current from one buffer in stage 1, moves current to the stage-1 forecast
+// and the forecast role to the spare buffer, and before stage 2 copies the starting state's points
+// outside the core into that buffer (device_copy_shell); every other value equals the reference's.
+static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullptr, bool share_end = false) {
     Ctx *c = as_ctx(ctx);
     const double steps[] = {1. / 3., 0.5, 1.0};
     const int NX = c->g.nx, NY = c->g.ny;
     const bool moist = c->g.microphysics != 0;
+    const int flast = moist ? DYN_QR : DYN_TH;  // the prognostics a step rotates
+    const bool shared = c->prog[DYN_U][DYN_ROLE_M] == c->prog[DYN_U][DYN_ROLE_CUR];
     int rc;
 #define RC(x)                      \
     do {                           \
@@ -650,6 +718,7 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
         if (hook) RC(hook(c, s, arg));
+        if (s == 1 && shared) RC(device_copy_shell(c, DYN_U, flast));
         RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageMomentum));
         if (two && (hipEventRecord(c->ev_fork, c->stream) != hipSuccess || hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess)) {
             set_error("fused step: fork onto the second stream failed");
@@ -670,8 +739,18 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
         }
         RC(rs);
         RC(rm);
-        if (moist && s < 2) RC(dyn_micro_advance_inner(ctx));
-        if (s < 2) RC(dyn_advance_inner(ctx));
+        // advance_inner / microphysics_advance_inner; from shared roles the forecast role moves to the
+        // spare buffer
+        if (s < 2)
+            for (int f = DYN_U; f <= flast; f++) {
+                if (c->prog[f][DYN_ROLE_M] == c->prog[f][DYN_ROLE_CUR]) {
+                    c->prog[f][DYN_ROLE_CUR] = c->prog[f][DYN_ROLE_P];
+                    c->prog[f][DYN_ROLE_P] = c->spare[f];
+                    c->spare[f] = nullptr;
+                } else {
+                    std::swap(c->prog[f][DYN_ROLE_CUR], c->prog[f][DYN_ROLE_P]);
+                }
+            }
     }
     if (moist) RC(dyn_kessler(ctx, 1, NX - 1, 1, NY - 1));
     // apply_boundary_condition_microphysics then apply_boundary_condition (different fields, so one
@@ -680,9 +759,18 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     RC(dyn_rayleigh(ctx, 1, NX - 1, 1, NY - 1, c->g.rayleigh_level));
     // microphysics_advance then advance_outer: Rayleigh damping leaves the moisture fields alone, so
     // the M <- P copies of every prognostic go in one launch after it
-    RC(device_copy_fields(c, DYN_U, moist ? DYN_QR : DYN_TH));
-    if (moist) RC(swap_roles(c, DYN_QV, DYN_QR));
-    RC(swap_roles(c, DYN_U, DYN_TH));
+    if (share_end) {  // M and current on the forecast buffer, the old M buffer spare
+        for (int f = DYN_U; f <= flast; f++) {
+            c->spare[f] = c->prog[f][DYN_ROLE_M];
+            c->prog[f][DYN_ROLE_M] = c->prog[f][DYN_ROLE_P];
+            c->prog[f][DYN_ROLE_P] = c->prog[f][DYN_ROLE_CUR];
+            c->prog[f][DYN_ROLE_CUR] = c->prog[f][DYN_ROLE_M];
+        }
+    } else {
+        RC(device_copy_fields(c, DYN_U, flast));
+        if (moist) RC(swap_roles(c, DYN_QV, DYN_QR));
+        RC(swap_roles(c, DYN_U, DYN_TH));
+    }
 #undef RC
     return DYN_OK;
 }
@@ -707,9 +795,13 @@ int dyn_step_fused(dyn_ctx *ctx, int nsteps) {
         int rc = ensure_rayleigh_table(c, c->g.rayleigh_level);
         if (rc) return rc;
     }
+    // M and current share a buffer between the steps of the call (no M <- P copy), not after its last
     for (int n = 0; n < nsteps; n++) {
-        int rc = fused_step(ctx);
-        if (rc) return rc;
+        int rc = fused_step(ctx, nullptr, nullptr, n < nsteps - 1);
+        if (rc) {
+            unshare_roles(c);
+            return rc;
+        }
     }
     return DYN_OK;
 }
@@ -725,8 +817,101 @@ int dyn_step_reference(dyn_ctx *ctx, int nsteps) {
     return DYN_OK;
 }
 
-// One step replayed from a hipGraph.  A step permutes the current/forecast buffers with period 2,
-// so the graph records two steps and an odd trailing step runs eagerly.
+// Graph-replayed fused steps.  A call of n >= 2 steps is a first step (roles apart -> M and current
+// sharing a buffer), (n-2) shared steps, and a last step (shared -> apart).  A shared step maps the
+// buffers (M = current, forecast, spare) = (X, Y, Z) to (Y, Z, X).  Each kind of piece is captured once
+// for each of the six orders of a field's three buffers, before the first call's first step (capture
+// launches nothing), so no call captures while its steps are timed.
+enum { kFirstStep = 0, kThreeShared, kOneShared, kLastStep, kStepKinds };
+
+static Ctx::StepGraph *find_step_graph(Ctx *c, int kind) {
+    for (auto &g : c->graphs)
+        if (g.kind == kind && !memcmp(g.key, c->prog, sizeof(g.key)) && !memcmp(g.spare, c->spare, sizeof(g.spare))) return &g;
+    return nullptr;
+}
+
+static int capture_step_graph(Ctx *c, int kind) {
+    dyn_ctx *ctx = reinterpret_cast<dyn_ctx *>(c);
+    Ctx::StepGraph ng{};
+    ng.kind = kind;
+    memcpy(ng.key, c->prog, sizeof(ng.key));
+    memcpy(ng.spare, c->spare, sizeof(ng.spare));
+    DYN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int rc = DYN_OK;
+    const int nsteps = kind == kThreeShared ? 3 : 1;
+    for (int n = 0; n < nsteps && !rc; n++) rc = fused_step(ctx, nullptr, nullptr, kind != kLastStep);
+    c->capturing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    memcpy(ng.post, c->prog, sizeof(ng.post));
+    memcpy(ng.post_spare, c->spare, sizeof(ng.post_spare));
+    memcpy(c->prog, ng.key, sizeof(ng.key));
+    memcpy(c->spare, ng.spare, sizeof(ng.spare));
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rc) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return rc ? rc : DYN_E_HIP;
+    }
+    ng.graph = graph;
+    DYN_HIP(hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0));
+    c->graphs.push_back(ng);
+    return DYN_OK;
+}
+
+// every kind for every order of the three buffers (the roles must be apart)
+static int ensure_step_graphs(Ctx *c) {
+    if (!c->graphs.empty()) return DYN_OK;
+    const int flast = c->g.microphysics ? DYN_QR : DYN_TH;
+    void *prog0[7][3], *spare0[7];
+    memcpy(prog0, c->prog, sizeof(prog0));
+    memcpy(spare0, c->spare, sizeof(spare0));
+    static const int perm[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
+    int rc = DYN_OK;
+    for (int q = 0; q < 6 && !rc; q++)
+        for (int kind = 0; kind < kStepKinds && !rc; kind++) {
+            for (int f = DYN_U; f <= flast; f++) {
+                void *t[3] = {prog0[f][DYN_ROLE_M], prog0[f][DYN_ROLE_CUR], prog0[f][DYN_ROLE_P]};
+                if (kind == kFirstStep) {  // apart: (M, current, forecast)
+                    c->prog[f][DYN_ROLE_M] = t[perm[q][0]];
+                    c->prog[f][DYN_ROLE_CUR] = t[perm[q][1]];
+                    c->prog[f][DYN_ROLE_P] = t[perm[q][2]];
+                    c->spare[f] = nullptr;
+                } else {  // shared: (M = current, forecast, spare)
+                    c->prog[f][DYN_ROLE_M] = c->prog[f][DYN_ROLE_CUR] = t[perm[q][0]];
+                    c->prog[f][DYN_ROLE_P] = t[perm[q][1]];
+                    c->spare[f] = t[perm[q][2]];
+                }
+            }
+            rc = capture_step_graph(c, kind);
+        }
+    memcpy(c->prog, prog0, sizeof(prog0));
+    memcpy(c->spare, spare0, sizeof(spare0));
+    if (rc) drop_graph(c);
+    return rc;
+}
+
+// times > 1 only for three shared steps, which restore the assignment
+static int replay_step_graph(Ctx *c, int kind, int times = 1) {
+    Ctx::StepGraph *sg = find_step_graph(c, kind);
+    if (!sg) {
+        set_error("internal: no step graph for the current buffer assignment");
+        return DYN_E_STATE;
+    }
+    if (times > 1 && (memcmp(sg->post, sg->key, sizeof(sg->key)) || memcmp(sg->post_spare, sg->spare, sizeof(sg->spare)))) {
+        set_error("internal: a repeated step graph must restore its buffer assignment");
+        return DYN_E_STATE;
+    }
+    for (int t = 0; t < times; t++) DYN_HIP(hipGraphLaunch(sg->exec, c->stream));
+    memcpy(c->prog, sg->post, sizeof(sg->post));
+    memcpy(c->spare, sg->post_spare, sizeof(sg->post_spare));
+    return DYN_OK;
+}
+
+// Steps replayed from hipGraphs.  Fused steps: the first step ends with M and current sharing a buffer,
+// the steps between run from graphs of three such steps, the last step separates the roles again with
+// the M <- P copy.  Per-operator steps (NZ the fused kernels do not take) permute the current/forecast
+// buffers with period 2, so their graph records two steps and an odd trailing step runs eagerly.
 int dyn_step(dyn_ctx *ctx, int nsteps) {
     Ctx *c = as_ctx(ctx);
     CHECK_CTX(c);
@@ -736,6 +921,16 @@ int dyn_step(dyn_ctx *ctx, int nsteps) {
     if (c->g.rayleigh_level < c->g.nz) {
         int rc = ensure_rayleigh_table(c, c->g.rayleigh_level);
         if (rc) return rc;
+    }
+    if (fused_supported(c)) {
+        if (nsteps == 1) return fused_step(ctx);
+        int rc = ensure_step_graphs(c);
+        if (!rc) rc = replay_step_graph(c, kFirstStep);
+        if (!rc && (nsteps - 2) / 3 > 0) rc = replay_step_graph(c, kThreeShared, (nsteps - 2) / 3);
+        for (int r = 0; r < (nsteps - 2) % 3 && !rc; r++) rc = replay_step_graph(c, kOneShared);
+        if (!rc) rc = replay_step_graph(c, kLastStep);
+        if (rc) unshare_roles(c);
+        return rc;
     }
     int pairs = nsteps / 2;
     if (pairs > 0) {
@@ -869,9 +1064,19 @@ int dyn_tune_get(dyn_ctx *ctx, int param) {
 
 namespace dyn {
 // one step in the fastest eager form, with a callback at the start of every stage
-int step_with_hook(Ctx *c, StageHook hook, void *arg) {
+// the M role back on the spare buffer after a sequence that stopped between fused steps (an error): the
+// contents are then those of a failed step, but every role has its own buffer again
+void unshare_roles(Ctx *c) {
+    for (int f = 0; f < 7; f++)
+        if (c->spare[f]) {
+            c->prog[f][DYN_ROLE_M] = c->spare[f];
+            c->spare[f] = nullptr;
+        }
+}
+
+int step_with_hook(Ctx *c, StageHook hook, void *arg, bool share_end) {
     dyn_ctx *ctx = reinterpret_cast<dyn_ctx *>(c);
-    return fused_supported(c) ? fused_step(ctx, hook, arg) : reference_step(ctx, hook, arg);
+    return fused_supported(c) ? fused_step(ctx, hook, arg, share_end) : reference_step(ctx, hook, arg);
 }
 }  // namespace dyn
 

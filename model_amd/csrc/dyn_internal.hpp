@@ -154,10 +154,25 @@ struct Ctx {
     int ray_level_cached = -1;
     int *dflag = nullptr;                             // device error flag (Kessler guard)
     PressurePlan pp;
-    // graph for dyn_step
+    // graph for dyn_step (per-operator steps: two steps restore the buffer assignment)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_parity = -1;
+    // Between the fused steps of one dyn_step / dyn_step_fused call the M and current roles of the
+    // stepped prognostics share a buffer (both hold the state at the start of the step); spare[f] is
+    // then the field's third buffer, nullptr otherwise.  Calls end with M and current apart again.
+    void *spare[7] = {nullptr};
+    // graphs of fused steps, keyed by their kind (first, three shared, one shared, last step of a call)
+    // and the buffer assignment they were captured with; post*: the assignment after them
+    struct StepGraph {
+        int kind;
+        void *key[7][3];
+        void *spare[7];
+        void *post[7][3];
+        void *post_spare[7];
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> graphs;
     AdjState *adj = nullptr;
     // launch-shape knobs (DYN_TUNE_*), 0 = default
     int tune[DYN_TUNE_COUNT] = {0};
@@ -202,7 +217,10 @@ int launch_convert(Ctx *c, void *dst, const double *src, size_t n, bool to_devic
 void adjoint_release(Ctx *c);
 // a whole RK3 step with a callback at the start of each stage (stage 0, 1, 2); a non-zero return aborts
 typedef int (*StageHook)(Ctx *c, int stage, void *arg);
-int step_with_hook(Ctx *c, StageHook hook, void *arg);
+// share_end: let the fused step end with M and current sharing a buffer (the next step must follow
+// through step_with_hook too, and the last one of a sequence must not share)
+int step_with_hook(Ctx *c, StageHook hook, void *arg, bool share_end = false);
+void unshare_roles(Ctx *c);
 bool fused_supported(const Ctx *c);
 int fused_waves(const Ctx *c);
 // ensembles: batched launches over a device table of members (dyn_fused.hip, dyn_pressure.hip)

@@ -281,6 +281,28 @@ def test_fused_launch_shapes_bitwise(precision, shape):
         assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), ids.FIELD_NAMES[f]
 
 
+@pytest.mark.parametrize("precision", [0, DYN_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("moist", [0, 1], ids=["dry", "moist"])
+def test_step_calls_of_any_length_bitwise(precision, moist):
+    """Between the steps of one call the fused step lets M and the current role share a buffer and
+    copies only the points stage 2 reads before writing; calls of every length, graph-replayed and
+    eager, in any sequence, leave every buffer of every role as the per-operator steps do."""
+    c = configs.supercell(nx=40, ny=36, nz=30) if moist else configs.vortex_in_shear(nx=40, ny=36, nz=30)
+    ref, g, e = (DynCore(c, precision=precision) for _ in range(3))
+    fields = (ids.U, ids.V, ids.W, ids.TH) + ((ids.QV, ids.QC, ids.QR) if moist else ())
+    for n in (1, 2, 5, 3, 7, 4, 8, 6):
+        ref.step_reference(n)
+        g.step(n)
+        e.step_fused(n)
+        for x in (ref, g, e):
+            x.sync()
+        for f in fields:
+            for role in (ids.ROLE_M, CUR, P):
+                want = ref.get_field(f, role)
+                assert np.array_equal(want, g.get_field(f, role)), ("graph", n, ids.FIELD_NAMES[f], role)
+                assert np.array_equal(want, e.get_field(f, role)), ("eager", n, ids.FIELD_NAMES[f], role)
+
+
 @pytest.mark.parametrize("nz", [66, 80, 128])
 @pytest.mark.parametrize("precision,moist", [(DYN_FP32, 1), (DYN_FP32, 0), (0, 1), (0, 0)],
                          ids=["fp32-moist", "fp32-dry", "fp64-moist", "fp64-dry"])
