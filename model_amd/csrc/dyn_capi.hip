@@ -817,41 +817,43 @@ int dyn_step_reference(dyn_ctx *ctx, int nsteps) {
     return DYN_OK;
 }
 
-// Graph-replayed fused steps.  A call of n >= 2 steps is a first step (roles apart -> M and current
-// sharing a buffer), (n-2) shared steps, and a last step (shared -> apart).  A shared step maps the
-// buffers (M = current, forecast, spare) = (X, Y, Z) to (Y, Z, X).  Each kind of piece is captured once
-// for each of the six orders of a field's three buffers, before the first call's first step (capture
-// launches nothing), so no call captures while its steps are timed.
-enum { kFirstStep = 0, kThreeShared, kOneShared, kLastStep, kStepKinds };
-
-static Ctx::StepGraph *find_step_graph(Ctx *c, int kind) {
+// Graph-replayed fused steps.  A run of k >= 3 steps, k a multiple of 3, whose first step ends with M
+// and current sharing a buffer, whose middle steps share, and whose last step separates the roles again
+// with the M <- P copy, maps the buffers (M, current, forecast) = (a, b, c) to (b, a, c) (a shared step
+// maps (M = current, forecast, spare) = (X, Y, Z) to (Y, Z, X)); M and current then hold the same
+// values, so naming them back (a, b, c) changes nothing.  Two steps with the roles apart also restore
+// (a, b, c).  Every call of n >= 2 steps is made of runs of 6, 3 and 2 steps, each one graph of the
+// current assignment; a context keeps its assignment from call to call, and single steps (which move
+// it) run eagerly.  The three graphs of an assignment are captured together when it is first met.
+static Ctx::StepGraph *find_step_graph(Ctx *c, int steps) {
     for (auto &g : c->graphs)
-        if (g.kind == kind && !memcmp(g.key, c->prog, sizeof(g.key)) && !memcmp(g.spare, c->spare, sizeof(g.spare))) return &g;
+        if (g.steps == steps && !memcmp(g.key, c->prog, sizeof(g.key))) return &g;
     return nullptr;
 }
 
-static int capture_step_graph(Ctx *c, int kind) {
+static int capture_step_graph(Ctx *c, int steps) {
     dyn_ctx *ctx = reinterpret_cast<dyn_ctx *>(c);
+    const int flast = c->g.microphysics ? DYN_QR : DYN_TH;
     Ctx::StepGraph ng{};
-    ng.kind = kind;
+    ng.steps = steps;
     memcpy(ng.key, c->prog, sizeof(ng.key));
-    memcpy(ng.spare, c->spare, sizeof(ng.spare));
     DYN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     int rc = DYN_OK;
-    const int nsteps = kind == kThreeShared ? 3 : 1;
-    for (int n = 0; n < nsteps && !rc; n++) rc = fused_step(ctx, nullptr, nullptr, kind != kLastStep);
+    for (int n = 0; n < steps && !rc; n++) rc = fused_step(ctx, nullptr, nullptr, steps > 2 && n < steps - 1);
     c->capturing = false;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
-    memcpy(ng.post, c->prog, sizeof(ng.post));
-    memcpy(ng.post_spare, c->spare, sizeof(ng.post_spare));
+    if (!rc && steps > 2)
+        for (int f = DYN_U; f <= flast; f++) std::swap(c->prog[f][DYN_ROLE_M], c->prog[f][DYN_ROLE_CUR]);
+    const bool back = !memcmp(ng.key, c->prog, sizeof(ng.key));
     memcpy(c->prog, ng.key, sizeof(ng.key));
-    memcpy(c->spare, ng.spare, sizeof(ng.spare));
-    if (rc || e != hipSuccess) {
+    for (auto &p : c->spare) p = nullptr;
+    if (rc || e != hipSuccess || !back) {
         if (graph) (void)hipGraphDestroy(graph);
-        if (!rc) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        return rc ? rc : DYN_E_HIP;
+        if (!rc && e != hipSuccess) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        if (!rc && e == hipSuccess) set_error("internal: a step graph did not restore the buffer assignment");
+        return rc ? rc : (e != hipSuccess ? DYN_E_HIP : DYN_E_STATE);
     }
     ng.graph = graph;
     DYN_HIP(hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0));
@@ -859,52 +861,20 @@ static int capture_step_graph(Ctx *c, int kind) {
     return DYN_OK;
 }
 
-// every kind for every order of the three buffers (the roles must be apart)
-static int ensure_step_graphs(Ctx *c) {
-    if (!c->graphs.empty()) return DYN_OK;
-    const int flast = c->g.microphysics ? DYN_QR : DYN_TH;
-    void *prog0[7][3], *spare0[7];
-    memcpy(prog0, c->prog, sizeof(prog0));
-    memcpy(spare0, c->spare, sizeof(spare0));
-    static const int perm[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
-    int rc = DYN_OK;
-    for (int q = 0; q < 6 && !rc; q++)
-        for (int kind = 0; kind < kStepKinds && !rc; kind++) {
-            for (int f = DYN_U; f <= flast; f++) {
-                void *t[3] = {prog0[f][DYN_ROLE_M], prog0[f][DYN_ROLE_CUR], prog0[f][DYN_ROLE_P]};
-                if (kind == kFirstStep) {  // apart: (M, current, forecast)
-                    c->prog[f][DYN_ROLE_M] = t[perm[q][0]];
-                    c->prog[f][DYN_ROLE_CUR] = t[perm[q][1]];
-                    c->prog[f][DYN_ROLE_P] = t[perm[q][2]];
-                    c->spare[f] = nullptr;
-                } else {  // shared: (M = current, forecast, spare)
-                    c->prog[f][DYN_ROLE_M] = c->prog[f][DYN_ROLE_CUR] = t[perm[q][0]];
-                    c->prog[f][DYN_ROLE_P] = t[perm[q][1]];
-                    c->spare[f] = t[perm[q][2]];
-                }
-            }
-            rc = capture_step_graph(c, kind);
-        }
-    memcpy(c->prog, prog0, sizeof(prog0));
-    memcpy(c->spare, spare0, sizeof(spare0));
-    if (rc) drop_graph(c);
-    return rc;
-}
-
-// times > 1 only for three shared steps, which restore the assignment
-static int replay_step_graph(Ctx *c, int kind, int times = 1) {
-    Ctx::StepGraph *sg = find_step_graph(c, kind);
+static int replay_steps(Ctx *c, int steps, int times) {
+    if (times <= 0) return DYN_OK;
+    if (!find_step_graph(c, 6)) {  // first meeting of this assignment: its three graphs
+        int rc = capture_step_graph(c, 6);
+        if (!rc) rc = capture_step_graph(c, 3);
+        if (!rc) rc = capture_step_graph(c, 2);
+        if (rc) return rc;
+    }
+    Ctx::StepGraph *sg = find_step_graph(c, steps);
     if (!sg) {
         set_error("internal: no step graph for the current buffer assignment");
         return DYN_E_STATE;
     }
-    if (times > 1 && (memcmp(sg->post, sg->key, sizeof(sg->key)) || memcmp(sg->post_spare, sg->spare, sizeof(sg->spare)))) {
-        set_error("internal: a repeated step graph must restore its buffer assignment");
-        return DYN_E_STATE;
-    }
     for (int t = 0; t < times; t++) DYN_HIP(hipGraphLaunch(sg->exec, c->stream));
-    memcpy(c->prog, sg->post, sizeof(sg->post));
-    memcpy(c->spare, sg->post_spare, sizeof(sg->post_spare));
     return DYN_OK;
 }
 
@@ -924,12 +894,19 @@ int dyn_step(dyn_ctx *ctx, int nsteps) {
     }
     if (fused_supported(c)) {
         if (nsteps == 1) return fused_step(ctx);
-        int rc = ensure_step_graphs(c);
-        if (!rc) rc = replay_step_graph(c, kFirstStep);
-        if (!rc && (nsteps - 2) / 3 > 0) rc = replay_step_graph(c, kThreeShared, (nsteps - 2) / 3);
-        for (int r = 0; r < (nsteps - 2) % 3 && !rc; r++) rc = replay_step_graph(c, kOneShared);
-        if (!rc) rc = replay_step_graph(c, kLastStep);
-        if (rc) unshare_roles(c);
+        // n = 6 q + (0 | 2 | 3 | 2+2 | 3+2), with 7 = 3+2+2 for r = 1
+        int q = nsteps / 6, r = nsteps % 6, n3 = 0, n2 = 0;
+        if (r == 1) {
+            q -= 1;
+            n3 = 1;
+            n2 = 2;
+        } else {
+            n3 = r == 3 || r == 5;
+            n2 = r == 2 || r == 5 ? 1 : (r == 4 ? 2 : 0);
+        }
+        int rc = replay_steps(c, 6, q);
+        if (!rc) rc = replay_steps(c, 3, n3);
+        if (!rc) rc = replay_steps(c, 2, n2);
         return rc;
     }
     int pairs = nsteps / 2;

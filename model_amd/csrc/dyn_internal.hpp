@@ -161,14 +161,11 @@ struct Ctx {
     // stepped prognostics share a buffer (both hold the state at the start of the step); spare[f] is
     // then the field's third buffer, nullptr otherwise.  Calls end with M and current apart again.
     void *spare[7] = {nullptr};
-    // graphs of fused steps, keyed by their kind (first, three shared, one shared, last step of a call)
-    // and the buffer assignment they were captured with; post*: the assignment after them
+    // graphs of 6, 3 or 2 fused steps that restore the buffer assignment they start from, keyed by
+    // that assignment
     struct StepGraph {
-        int kind;
+        int steps;
         void *key[7][3];
-        void *spare[7];
-        void *post[7][3];
-        void *post_spare[7];
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
