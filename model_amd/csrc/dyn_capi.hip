@@ -43,7 +43,7 @@ bool fused_supported(const Ctx *c);
 int fused_prepare(Ctx *c);
 int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh, int parts);
 int launch_zero_moisture_shell(Ctx *c, int il, int ih, int jl, int jh);
-int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw);
+int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level = -1);
 
 namespace {
 
@@ -755,8 +755,16 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     if (moist) RC(dyn_kessler(ctx, 1, NX - 1, 1, NY - 1));
     // apply_boundary_condition_microphysics then apply_boundary_condition (different fields, so one
     // kernel serves both; the moisture pointers swap only afterwards)
-    RC(launch_post_stage(c, 0.0, false, true, false, true));
-    RC(dyn_rayleigh(ctx, 1, NX - 1, 1, NY - 1, c->g.rayleigh_level));
+    // and rayleigh_damping(1, NX-1, 1, NY-1) in the same kernels: it damps each written value pointwise
+    const int lev = c->g.rayleigh_level;
+    if (lev < c->g.nz) {
+        if (lev < 0 || lev == c->g.nz - 1) {
+            set_error("rayleigh_damping: level must be in [0, NZ-2] or >= NZ (NZ-1 divides 0 by 0)");
+            return DYN_E_ARG;
+        }
+        RC(ensure_rayleigh_table(c, lev));
+    }
+    RC(launch_post_stage(c, 0.0, false, true, false, true, lev < c->g.nz ? lev : -1));
     // microphysics_advance then advance_outer: Rayleigh damping leaves the moisture fields alone, so
     // the M <- P copies of every prognostic go in one launch after it
     if (share_end) {  // M and current on the forecast buffer, the old M buffer spare

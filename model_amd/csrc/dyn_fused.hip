@@ -827,6 +827,10 @@ struct PostArgs {
     const double *cs, *cn;
     int js, jn;
     int zl, zh, zjl, zjh;  // zero_moisture bounds (qv clip)
+    // Rayleigh damping after the boundary update (end of a step): levels >= ray_level, coefficients
+    // [u v theta][k] then [w][k]; ray_level >= NZ: none
+    const double *ray = nullptr;
+    int ray_level = 1 << 30;
 };
 
 template <typename T>
@@ -837,16 +841,20 @@ __device__ __forceinline__ T sponge_coef(const Geo &g, const PostArgs &a, int j)
 }
 
 template <typename T>
-__device__ __forceinline__ void write_col(const Geo &g, T *A, int isrc, int j, int k, T fin, T prelid) {
+// damp: rayleigh_damping (damping.cpp:1148-1177) of the written values with coefficient cd on the
+// columns it covers, i in [1, NX-1) (the caller checks j and k)
+__device__ __forceinline__ void write_col(const Geo &g, T *A, int isrc, int j, int k, T fin, T prelid, bool damp = false,
+                                          T cd = T(0)) {
     long long s = g.NYNZ;
     long long p = (long long)isrc * s + (long long)j * g.NZ + k;
-    A[p] = fin;
+    auto dv = [&](int i, T v) { return damp && i >= 1 && i < g.NX - 1 ? v - cd * v : v; };
+    A[p] = dv(isrc, fin);
     const int shift = g.NX - 6;
     if (isrc >= shift && isrc < shift + 3) {
         int ih = isrc - shift;  // halo 0, 1, 2
-        A[p - (long long)shift * s] = ih == 0 ? prelid : fin;
+        A[p - (long long)shift * s] = dv(ih, ih == 0 ? prelid : fin);
     }
-    if (isrc >= 3 && isrc < 6) A[p + (long long)shift * s] = fin;
+    if (isrc >= 3 && isrc < 6) A[p + (long long)shift * s] = dv(isrc + shift, fin);
 }
 
 // PGF (solver.cpp:123-135) + sponge/periodic/lid for UP, VP and WP
@@ -881,9 +889,12 @@ __device__ __forceinline__ void post_uvw_col(const Geo &g, const Cols &c, const 
         if (k == NZ - 1) { uf = uz; vf = vz; }
         if (k == 0 || k == 1 || k == NZ - 1) wf = T(0);
     }
-    write_col(g, UP, isrc, j, k, uf, u);
-    write_col(g, VP, isrc, j, k, vf, v);
-    write_col(g, WP, isrc, j, k, wf, w);
+    // Rayleigh damping after the boundary update (j in [1, NY-1), k >= level)
+    const bool dm = k >= a.ray_level && j >= 1 && j < g.NY - 1;
+    const T cu = dm ? T(a.ray[k]) : T(0), cw = dm ? T(a.ray[NZ + k]) : T(0);
+    write_col(g, UP, isrc, j, k, uf, u, dm, cu);
+    write_col(g, VP, isrc, j, k, vf, v, dm, cu);
+    write_col(g, WP, isrc, j, k, wf, w, dm, cw);
 }
 
 // sponge/periodic/lid for THP (and QVP, QCP, QRP with zero_moisture outside the advected box); only the
@@ -903,12 +914,26 @@ __device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, co
     const bool full = coef >= T(0) || j < 3 || j >= g.NY - 3 || isrc < 6 || isrc >= g.NX - 6;
     constexpr int NFLD = MOIST ? 4 : 1;
     if (!full) {  // interior column: lid only (values at 1 and NZ-2 are final already)
+        // with Rayleigh damping (end of a step) theta is rewritten from level ray_level up as well: the
+        // lid from the undamped levels 1 and NZ-2, then the damping (j in [3, NY-3) here)
+        const bool ray = a.ray_level < NZ;
         if (lid_interior && j >= 1 && (k == 0 || k == NZ - 1)) {
             int ks = k == 0 ? 1 : NZ - 2;
 #pragma unroll
-            for (int f = 0; f < NFLD; f++) {
+            for (int f = ray ? 1 : 0; f < NFLD; f++) {
                 T *A = (T *)a.f[3 + f];
                 A[col + k] = A[col + ks];
+            }
+        }
+        if (ray) {
+            T *A = (T *)a.f[3];
+            const T x = A[col + kc];
+            const T x1 = __shfl(x, 1), xz = __shfl(x, NZ - 2);
+            const bool lid = lid_interior && (k == 0 || k == NZ - 1);
+            if (k < NZ && (lid || k >= a.ray_level)) {
+                T fin = k == 0 && lid ? x1 : (k == NZ - 1 && lid ? xz : x);
+                if (k >= a.ray_level) fin = fin - T(a.ray[k]) * fin;
+                A[col + k] = fin;
             }
         }
         return;
@@ -931,7 +956,9 @@ __device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, co
                 if (k == 0) fin = x1;
                 if (k == NZ - 1) fin = xz;
             }
-            write_col(g, A, isrc, j, k, fin, x);
+            // Rayleigh damping of theta after its boundary update (j in [1, NY-1), k >= level)
+            const bool dm = f == 0 && k >= a.ray_level && j >= 1 && j < g.NY - 1;
+            write_col(g, A, isrc, j, k, fin, x, dm, dm ? T(a.ray[k]) : T(0));
         }
     }
 }
@@ -1063,8 +1090,12 @@ __global__ void __launch_bounds__(256) k_ens_copy(const EnsMember *mem, int nmem
 }
 
 template <typename T>
-int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw) {
+int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level) {
     PostArgs a{};
+    if (ray_level >= 0 && ray_level < c->g.nz) {
+        a.ray = c->raycoef;
+        a.ray_level = ray_level;
+    }
     for (int f = 0; f < 7; f++) a.f[f] = c->prog[f][DYN_ROLE_P];
     a.pi = c->pi;
     a.qb = c->bar[4];
@@ -1217,9 +1248,9 @@ int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh, int
 
 // PGF (optional) and the boundary update of every prognostic after launch_fused_advect; stands for
 // dyn_pgf + zero_moisture's remainder + dyn_apply_bc + dyn_apply_bc_micro of one stage
-int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw) {
-    if (c->prec == DYN_FP32) return post_t<float>(c, step, pgf, scalars, clip, uvw);
-    return post_t<double>(c, step, pgf, scalars, clip, uvw);
+int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level) {
+    if (c->prec == DYN_FP32) return post_t<float>(c, step, pgf, scalars, clip, uvw, ray_level);
+    return post_t<double>(c, step, pgf, scalars, clip, uvw, ray_level);
 }
 
 // the rest of zero_moisture(il,ih,jl,jh) after launch_fused_advect on the box [3,NX-3) x [3,NY-3)

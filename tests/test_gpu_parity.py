@@ -286,8 +286,13 @@ def test_fused_launch_shapes_bitwise(precision, shape):
 def test_step_calls_of_any_length_bitwise(precision, moist):
     """Between the steps of one call the fused step lets M and the current role share a buffer and
     copies only the points stage 2 reads before writing; calls of every length, graph-replayed and
-    eager, in any sequence, leave every buffer of every role as the per-operator steps do."""
-    c = configs.supercell(nx=40, ny=36, nz=30) if moist else configs.vortex_in_shear(nx=40, ny=36, nz=30)
+    eager, in any sequence, leave every buffer of every role as the per-operator steps do.  Both cases
+    damp the upper levels (Rayleigh damping inside the step-end boundary kernels)."""
+    if moist:
+        c = configs.supercell(nx=40, ny=36, nz=56)  # 14 km deep: damping above 12 km
+    else:
+        c = configs.small_case(nx=40, ny=36, nz=30, microphysics=0, rayleigh_level=20, amplitude=0.5)
+    assert c.rayleigh_level < c.nz
     ref, g, e = (DynCore(c, precision=precision) for _ in range(3))
     fields = (ids.U, ids.V, ids.W, ids.TH) + ((ids.QV, ids.QC, ids.QR) if moist else ())
     for n in (1, 2, 5, 3, 7, 4, 8, 6):
