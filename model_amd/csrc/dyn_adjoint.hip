@@ -469,9 +469,18 @@ struct AdjState {
     int nsteps = 0, every = 0;
     size_t field_bytes = 0;
     std::vector<void *> snap;  // snapshot s, field f, role r at snap[(s * 4 + f) * 3 + r]; last = the final state
-    std::vector<void *> rec;   // segment step n, stage st, field f at rec[(n * 3 + st) * 4 + f]
-    std::vector<void *> slabs; // allocations snap and rec are carved from
-    int rec_seg = -1;          // segment whose stage states rec holds (-1: none)
+    // two records of a segment's stage states: step n, stage st, field f at rec[b][(n * 3 + st) * 4 + f];
+    // the second is allocated by the reverse sweep when there is more than one segment, so that the
+    // next segment down is recomputed into one while the adjoint steps read the other
+    std::vector<void *> rec[2];
+    std::vector<void *> slabs;   // allocations snap and rec are carved from
+    int rec_seg[2] = {-1, -1};   // segment whose stage states rec[b] holds (-1: none)
+    int rec_w = 0, rec_r = 0;    // record written by the stage hook / read by the adjoint steps
+    // the reverse sweep's adjoint steps run on their own stream with their own spectrum scratch
+    hipStream_t astream = nullptr;
+    hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr};
+    bool adj_pending[2] = {false, false};  // ev_adj[b]: adjoint steps that read rec[b] were queued
+    void *spec_a = nullptr, *spec2_a = nullptr;
     void *adj[kAdjFields][3] = {{nullptr}};
     void *pihat = nullptr, *r1 = nullptr, *r2 = nullptr;
     void *faces[18] = {nullptr};  // per-face adjoint weights of the advection transpose (Aw, G per kind)
@@ -490,7 +499,11 @@ void adj_free(Ctx *c) {
     for (auto &f : s->adj)
         for (void *p : f)
             if (p) (void)hipFree(p);
-    void *ptrs[] = {s->pihat, s->r1, s->r2, s->stage_dbl};
+    if (s->astream) (void)hipStreamSynchronize(s->astream);
+    for (hipEvent_t e : {s->ev_rec[0], s->ev_rec[1], s->ev_adj[0], s->ev_adj[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (s->astream) (void)hipStreamDestroy(s->astream);
+    void *ptrs[] = {s->pihat, s->r1, s->r2, s->stage_dbl, s->spec_a, s->spec2_a};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (void *p : s->faces)
@@ -537,7 +550,7 @@ int record_hook(Ctx *c, int stage, void *arg) {
     CopyList cl{};
     for (int f = 0; f < kAdjFields; f++) {
         cl.src[f] = c->prog[kAdjIds[f]][DYN_ROLE_CUR];
-        cl.dst[f] = s->rec[((size_t)n * 3 + stage) * 4 + f];
+        cl.dst[f] = s->rec[s->rec_w][((size_t)n * 3 + stage) * 4 + f];
     }
     return launch_copy_list(c, cl, kAdjFields, s->field_bytes);
 }
@@ -587,10 +600,11 @@ int stage_adjoint(Ctx *c, int n, int stage, double step) {
     }
     AdvAdjArgs<T> a{};
     const size_t rb = ((size_t)(n - s->rec_base) * 3 + stage) * 4;
-    a.U = (const T *)s->rec[rb + 0];
-    a.V = (const T *)s->rec[rb + 1];
-    a.W = (const T *)s->rec[rb + 2];
-    a.TH = (const T *)s->rec[rb + 3];
+    const std::vector<void *> &rec = s->rec[s->rec_r];
+    a.U = (const T *)rec[rb + 0];
+    a.V = (const T *)rec[rb + 1];
+    a.W = (const T *)rec[rb + 2];
+    a.TH = (const T *)rec[rb + 3];
     a.UB = (const T *)c->bar[0];
     a.VB = (const T *)c->bar[1];
     a.WB = (const T *)c->bar[2];
@@ -660,6 +674,26 @@ int step_adjoint(Ctx *c, int n) {
     return DYN_OK;
 }
 
+// the second record, the adjoint stream, its events and its spectrum scratch (once per record pass); on
+// failure the sweep runs serially
+int ensure_pipeline(Ctx *c) {
+    AdjState *s = c->adj;
+    if (s->astream) return DYN_OK;
+    const size_t spec_bytes = sizeof(double) * 2 * (size_t)c->pp.ni * c->pp.nj * c->pp.nk2;
+    if (alloc_fields(s, s->rec[1], (size_t)s->every * 12, s->field_bytes) != DYN_OK) {
+        set_error("");
+        return DYN_E_HIP;
+    }
+    if (hipMalloc(&s->spec_a, spec_bytes) != hipSuccess || (c->pp.spec2 && hipMalloc(&s->spec2_a, spec_bytes) != hipSuccess)) {
+        (void)hipGetLastError();
+        return DYN_E_HIP;
+    }
+    for (hipEvent_t *e : {&s->ev_rec[0], &s->ev_rec[1], &s->ev_adj[0], &s->ev_adj[1]})
+        DYN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    DYN_HIP(hipStreamCreate(&s->astream));
+    return DYN_OK;
+}
+
 template <typename T>
 int backward_t(Ctx *c, const double *const *seed, double *const *grad) {
     AdjState *s = c->adj;
@@ -681,27 +715,78 @@ int backward_t(Ctx *c, const double *const *seed, double *const *grad) {
         DYN_LAUNCH_CHECK();
     }
     const int nseg = (s->nsteps + s->every - 1) / s->every;
-    for (int seg = nseg - 1; seg >= 0; seg--) {
+    // recompute segment seg from its snapshot into record b (on the context's streams)
+    auto recompute = [&](int seg, int b) -> int {
         const int n0 = seg * s->every, n1 = std::min(s->nsteps, n0 + s->every);
-        int rc = DYN_OK;
-        if (s->rec_seg != seg) {  // the record pass keeps the last segment: recompute only the others
-            rc = copy_state(c, nullptr, &s->snap[(size_t)seg * 12], false);
-            if (rc) return rc;
-            for (int n = n0; n < n1; n++) {
-                int local = n - n0;
-                rc = step_with_hook(c, record_hook, &local, n < n1 - 1);
-                if (rc) {
-                    unshare_roles(c);
-                    return rc;
-                }
+        int rc = copy_state(c, nullptr, &s->snap[(size_t)seg * 12], false);
+        if (rc) return rc;
+        s->rec_w = b;
+        for (int n = n0; n < n1; n++) {
+            int local = n - n0;
+            rc = step_with_hook(c, record_hook, &local, n < n1 - 1);
+            if (rc) {
+                unshare_roles(c);
+                return rc;
             }
-            s->rec_seg = seg;
         }
+        s->rec_seg[b] = seg;
+        return DYN_OK;
+    };
+    // the adjoint steps of segment seg from record b, on `stream` with spectrum scratch spec/spec2
+    auto adjoint_steps = [&](int seg, int b, hipStream_t stream, void *spec, void *spec2) -> int {
+        const int n0 = seg * s->every, n1 = std::min(s->nsteps, n0 + s->every);
+        hipStream_t own = c->stream;
+        void *sp = c->pp.spec, *sp2 = c->pp.spec2;
+        c->stream = stream;
+        c->pp.spec = spec;
+        c->pp.spec2 = spec2;
+        s->rec_r = b;
         s->rec_base = n0;
-        for (int n = n1 - 1; n >= n0; n--) {
-            rc = step_adjoint<T>(c, n);
+        int rc = DYN_OK;
+        for (int n = n1 - 1; n >= n0 && !rc; n--) rc = step_adjoint<T>(c, n);
+        c->stream = own;
+        c->pp.spec = sp;
+        c->pp.spec2 = sp2;
+        return rc;
+    };
+    const bool pipe = nseg > 1 && ensure_pipeline(c) == DYN_OK;
+    if (!pipe) {
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            int rc = DYN_OK;
+            if (s->rec_seg[0] != seg) rc = recompute(seg, 0);  // the record pass keeps the last segment
+            if (!rc) rc = adjoint_steps(seg, 0, c->stream, c->pp.spec, c->pp.spec2);
             if (rc) return rc;
         }
+    } else {
+        // Segment seg's adjoint steps run on the adjoint stream while segment seg-1 is recomputed on the
+        // context's streams into the other record; events order each record's writes and reads.
+        DYN_HIP(hipEventRecord(s->ev_rec[0], c->stream));  // the seed and the record pass come first
+        DYN_HIP(hipStreamWaitEvent(s->astream, s->ev_rec[0], 0));
+        s->adj_pending[0] = s->adj_pending[1] = false;
+        int b = -1;
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            b = s->rec_seg[0] == seg ? 0 : (s->rec_seg[1] == seg ? 1 : -1);
+            if (b < 0) {  // not recorded (a second sweep over one record pass)
+                b = 0;
+                if (s->adj_pending[b]) DYN_HIP(hipStreamWaitEvent(c->stream, s->ev_adj[b], 0));
+                int rc = recompute(seg, b);
+                if (rc) return rc;
+                DYN_HIP(hipEventRecord(s->ev_rec[b], c->stream));
+            }
+            if (seg > 0) {
+                const int ob = 1 - b;
+                if (s->adj_pending[ob]) DYN_HIP(hipStreamWaitEvent(c->stream, s->ev_adj[ob], 0));
+                int rc = recompute(seg - 1, ob);
+                if (rc) return rc;
+                DYN_HIP(hipEventRecord(s->ev_rec[ob], c->stream));
+            }
+            DYN_HIP(hipStreamWaitEvent(s->astream, s->ev_rec[b], 0));
+            int rc = adjoint_steps(seg, b, s->astream, s->spec_a, s->spec2_a);
+            if (rc) return rc;
+            DYN_HIP(hipEventRecord(s->ev_adj[b], s->astream));
+            s->adj_pending[b] = true;
+        }
+        DYN_HIP(hipStreamWaitEvent(c->stream, s->ev_adj[b], 0));  // the gradients follow the last adjoint step
     }
     // the initial fields are set into all three buffers: their gradient is the sum of the three adjoints
     for (int f = 0; f < kAdjFields; f++) {
@@ -769,7 +854,7 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
     s->every = std::min(every, nsteps);
     const int nseg = (nsteps + s->every - 1) / s->every;
     int rc = alloc_fields(s, s->snap, (size_t)(nseg + 1) * 12, s->field_bytes);
-    if (!rc) rc = alloc_fields(s, s->rec, (size_t)s->every * 12, s->field_bytes);
+    if (!rc) rc = alloc_fields(s, s->rec[0], (size_t)s->every * 12, s->field_bytes);
     if (rc) {
         adj_free(c);
         return rc;
@@ -800,7 +885,7 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
             return rc;
         }
     }
-    s->rec_seg = nseg - 1;
+    s->rec_seg[0] = nseg - 1;
     rc = copy_state(c, &s->snap[(size_t)nseg * 12], nullptr, true);
     if (rc) return rc;
     DYN_HIP(hipStreamSynchronize(c->stream));
