@@ -102,12 +102,15 @@ int dyn_micro_advance_outer(dyn_ctx *ctx);
 /* ---- whole steps ------------------------------------------------------------------------- */
 /* s_integrate_rk3 (driver_serial.cpp:178-286) as the sequence of per-operator calls above */
 int dyn_step_reference(dyn_ctx *ctx, int nsteps);
-/* the step replayed from a captured hipGraph: when NZ <= 64 or 66 <= NZ <= 128 the stage advection runs
- * as the fused momentum and scalar kernels with fused boundary epilogues (same values as the per-operator
- * kernels, bit for bit, in either precision); otherwise the per-operator sequence is replayed */
+/* nsteps steps replayed from captured hipGraphs: when NZ <= 64 or 66 <= NZ <= 128 the stage advection
+ * runs as the fused momentum and scalar kernels with fused boundary epilogues (same values as the
+ * per-operator kernels, bit for bit, in either precision); otherwise the per-operator sequence is
+ * replayed.  Between the steps of one call the M and current roles share a buffer (no M <- P copy), so
+ * one call of n steps is cheaper than n calls of one; every call ends with the three buffers apart and
+ * the values n single steps leave. */
 int dyn_step(dyn_ctx *ctx, int nsteps);
-/* the fused-stage step launched eagerly (either precision; needs NZ <= 64 or 66 <= NZ <= 128, and
- * fields below 4 GiB) */
+/* the fused-stage step launched eagerly, with the same sharing between the steps of the call (either
+ * precision; needs NZ <= 64 or 66 <= NZ <= 128, and fields below 4 GiB) */
 int dyn_step_fused(dyn_ctx *ctx, int nsteps);
 
 /* ---- measurement ------------------------------------------------------------------------- */
