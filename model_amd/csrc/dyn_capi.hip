@@ -43,7 +43,9 @@ bool fused_supported(const Ctx *c);
 int fused_prepare(Ctx *c);
 int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh, int parts);
 int launch_zero_moisture_shell(Ctx *c, int il, int ih, int jl, int jh);
-int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level = -1);
+// interior_done: the step-end update of the columns it changes only at the lid is done (by Kessler)
+int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level = -1,
+                      bool interior_done = false);
 
 namespace {
 
@@ -752,10 +754,11 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
                 }
             }
     }
-    if (moist) RC(dyn_kessler(ctx, 1, NX - 1, 1, NY - 1));
-    // apply_boundary_condition_microphysics then apply_boundary_condition (different fields, so one
-    // kernel serves both; the moisture pointers swap only afterwards)
-    // and rayleigh_damping(1, NX-1, 1, NY-1) in the same kernels: it damps each written value pointwise
+    // run_microphysics(1, NX-1, 1, NY-1); apply_boundary_condition_microphysics then
+    // apply_boundary_condition (different fields, so one kernel serves both; the moisture pointers swap
+    // only afterwards); rayleigh_damping(1, NX-1, 1, NY-1) in the same kernels, since it damps each
+    // written value pointwise.  Moist steps leave the columns the boundary update changes only at the
+    // lid (and by theta's damping) to Kessler's threads, and the boundary kernel does the rest.
     const int lev = c->g.rayleigh_level;
     if (lev < c->g.nz) {
         if (lev < 0 || lev == c->g.nz - 1) {
@@ -764,7 +767,9 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
         }
         RC(ensure_rayleigh_table(c, lev));
     }
-    RC(launch_post_stage(c, 0.0, false, true, false, true, lev < c->g.nz ? lev : -1));
+    const int ray = lev < c->g.nz ? lev : -1;
+    if (moist) RC(launch_kessler_step_end(c, ray));
+    RC(launch_post_stage(c, 0.0, false, true, false, true, ray, moist));
     // microphysics_advance then advance_outer: Rayleigh damping leaves the moisture fields alone, so
     // the M <- P copies of every prognostic go in one launch after it
     if (share_end) {  // M and current on the forecast buffer, the old M buffer spare

@@ -1090,7 +1090,7 @@ __global__ void __launch_bounds__(256) k_ens_copy(const EnsMember *mem, int nmem
 }
 
 template <typename T>
-int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level) {
+int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level, bool interior_done) {
     PostArgs a{};
     if (ray_level >= 0 && ray_level < c->g.nz) {
         a.ray = c->raycoef;
@@ -1120,7 +1120,7 @@ int post_t(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int
         // after a stage the advection kernel has set the interior lids; at the end of the step (no
         // gradient force) Kessler has changed the interior, so they are redone
         const bool lid_interior = !pgf;
-        if (!lid_interior) {  // only the sponge rows, ghost rows and periodic source columns change
+        if (!lid_interior || interior_done) {  // only the sponge rows, ghost rows and periodic source columns change
             const ShellCols sc = shell_cols(c);
             const dim3 gs((unsigned)std::max(1, (sc.total + rows - 1) / rows));
             if (c->g.microphysics)
@@ -1248,9 +1248,9 @@ int launch_fused_advect(Ctx *c, double step, int il, int ih, int jl, int jh, int
 
 // PGF (optional) and the boundary update of every prognostic after launch_fused_advect; stands for
 // dyn_pgf + zero_moisture's remainder + dyn_apply_bc + dyn_apply_bc_micro of one stage
-int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level) {
-    if (c->prec == DYN_FP32) return post_t<float>(c, step, pgf, scalars, clip, uvw, ray_level);
-    return post_t<double>(c, step, pgf, scalars, clip, uvw, ray_level);
+int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level, bool interior_done) {
+    if (c->prec == DYN_FP32) return post_t<float>(c, step, pgf, scalars, clip, uvw, ray_level, interior_done);
+    return post_t<double>(c, step, pgf, scalars, clip, uvw, ray_level, interior_done);
 }
 
 // the rest of zero_moisture(il,ih,jl,jh) after launch_fused_advect on the box [3,NX-3) x [3,NY-3)

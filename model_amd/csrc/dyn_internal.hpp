@@ -204,6 +204,7 @@ int launch_pgf(Ctx *c, double step, int il, int ih, int jl, int jh);
 int launch_bc(Ctx *c, bool micro);
 int launch_periodic(Ctx *c, void *const *fields, int nfields);
 int launch_kessler(Ctx *c, int il, int ih, int jl, int jh);
+int launch_kessler_step_end(Ctx *c, int ray_level);
 int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level);
 int rayleigh_table(Ctx *c, int level);
 int pressure_init(Ctx *c);

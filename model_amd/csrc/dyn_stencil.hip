@@ -399,11 +399,25 @@ __global__ void k_upper_lower(Geo g, FieldList fl, int nfill) {
 }
 
 // ---- Kessler warm rain, kessler.cpp:25-183 -------------------------------------------------------
+// The step-end boundary update of theta and moisture in the columns it changes only at the lid, done
+// by Kessler's threads of levels 1 and NZ-2 (the lid takes their values), with theta's Rayleigh damping
+// from level `lev` up (after the lid, as the reference orders them): columns i in [i0, i1), j in [j0, j1).
+// The thread of level 1 also adds the column's precip_rate (it reads level 0 before the lid overwrites
+// it), and k_precip skips those columns.
+struct KessTail {
+    int on = 0;
+    int i0 = 0, i1 = 0, j0 = 0, j1 = 0;
+    int lev = 1 << 30;
+    const double *ray = nullptr;
+    const void *vt = nullptr;
+    void *acc = nullptr;
+    __device__ bool in(int i, int j) const { return on && i >= i0 && i < i1 && j >= j0 && j < j1; }
+};
 template <typename T>
 __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ THP, T *__restrict__ QVP, T *__restrict__ QCP,
                                                  T *__restrict__ QRP, const T *__restrict__ THB, const T *__restrict__ QB,
                                                  const T *__restrict__ PB, const T *__restrict__ PI, int *__restrict__ flag, int il,
-                                                 int jl, int jh) {
+                                                 int jl, int jh, KessTail tail) {
     int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y * blockDim.y + threadIdx.y + jl;
     int i = blockIdx.z + il;
@@ -450,6 +464,25 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     qv -= C;
     T dc = (Lv / (cp * pressure)) * C;
     th += dc;
+    if (tail.in(i, j)) {
+        auto damp = [&](int kk, T v) { return kk >= tail.lev ? v - T(tail.ray[kk]) * v : v; };
+        const int kl = k == 1 ? 0 : (k == g.NZ - 2 ? g.NZ - 1 : -1);  // the lid level this level feeds
+        if (k == 1) {  // precip_rate (microphysics.cpp:143-160) with level 0 before the lid
+            const T *VT = (const T *)tail.vt;
+            const long long p0 = p - 1;
+            T *acc = (T *)tail.acc;
+            const T q0 = QRP[p0];
+            acc[(long long)i * g.NY + j] += T(c.rhow[1]) * T(0.5) * (VT[p0] + VT[p0 + 1]) * T(0.5) * (q0 + qr) * T(g.dt);
+        }
+        if (kl >= 0) {
+            const long long pl = p + (kl - k);
+            THP[pl] = damp(kl, th);
+            QVP[pl] = qv;
+            QCP[pl] = qc;
+            QRP[pl] = qr;
+        }
+        th = damp(k, th);
+    }
     THP[p] = th;
     QVP[p] = qv;
     QCP[p] = qc;
@@ -459,10 +492,10 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
 // precip_rate, microphysics.cpp:143-160
 template <typename T>
 __global__ void k_precip(Geo g, Cols c, const T *__restrict__ VT, const T *__restrict__ QRP, T *__restrict__ acc, int il, int ih,
-                         int jl, int jh) {
+                         int jl, int jh, KessTail tail) {
     int j = blockIdx.x * blockDim.x + threadIdx.x + jl;
     int i = blockIdx.y + il;
-    if (j >= jh || i >= ih) return;
+    if (j >= jh || i >= ih || tail.in(i, j)) return;
     long long p = IX(i, j, 0);
     acc[(long long)i * g.NY + j] += T(c.rhow[1]) * T(0.5) * (VT[p] + VT[p + 1]) * T(0.5) * (QRP[p] + QRP[p + 1]) * T(g.dt);
 }
@@ -630,19 +663,20 @@ int bc_t(Ctx *c, bool micro) {
 }
 
 template <typename T>
-int kessler_t(Ctx *c, int il, int ih, int jl, int jh) {
+int kessler_t(Ctx *c, int il, int ih, int jl, int jh, KessTail tail = KessTail{}) {
     dim3 blk, grd = col_grid(c->geo, c->geo.NZ - 2, jl, jh, il, ih, blk);
     auto P = [&](int f) { return (T *)c->prog[f][2]; };
     {
         Timed _t(c, "kessler", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
         hipLaunchKernelGGL(k_kessler<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_TH), P(DYN_QV), P(DYN_QC), P(DYN_QR),
-                       (const T *)c->bar[3], (const T *)c->bar[4], (const T *)c->bar[5], (const T *)c->pi, c->dflag, il, jl, jh);
+                       (const T *)c->bar[3], (const T *)c->bar[4], (const T *)c->bar[5], (const T *)c->pi, c->dflag, il, jl, jh,
+                           tail);
     }
     DYN_LAUNCH_CHECK();
     {
         Timed _t(c, "precip", (double)(ih - il) * (jh - jl));
         hipLaunchKernelGGL(k_precip<T>, dim3((jh - jl + 63) / 64, ih - il), dim3(64), 0, c->stream, c->geo, c->cols, (const T *)c->vt,
-                       P(DYN_QR), (T *)c->accrain, il, ih, jl, jh);
+                       P(DYN_QR), (T *)c->accrain, il, ih, jl, jh, tail);
     }
     DYN_LAUNCH_CHECK();
     return DYN_OK;
@@ -674,7 +708,25 @@ int launch_zero_moisture(Ctx *c, int il, int ih, int jl, int jh) { return DISPAT
 int launch_pgf(Ctx *c, double step, int il, int ih, int jl, int jh) { return DISPATCH(c, pgf_t, c, step, il, ih, jl, jh); }
 int launch_bc(Ctx *c, bool micro) { return DISPATCH(c, bc_t, c, micro); }
 int launch_periodic(Ctx *c, void *const *fields, int nfields) { return DISPATCH(c, periodic_t, c, fields, nfields); }
-int launch_kessler(Ctx *c, int il, int ih, int jl, int jh) { return DISPATCH(c, kessler_t, c, il, ih, jl, jh); }
+int launch_kessler(Ctx *c, int il, int ih, int jl, int jh) { return DISPATCH(c, kessler_t, c, il, ih, jl, jh, KessTail{}); }
+// Kessler over [1, NX-1) x [1, NY-1) with the lid and theta's damping (ray_level < 0: none) of the columns
+// the step-end boundary update would change only at the lid: i in [6, NX-6), rows between the sponge
+// and ghost rows
+int launch_kessler_step_end(Ctx *c, int ray_level) {
+    KessTail t;
+    t.on = 1;
+    t.i0 = 6;
+    t.i1 = c->g.nx - 6;
+    t.j0 = std::min(c->g.ny, std::max(c->g.sponge_south, 3));
+    t.j1 = c->g.ny - std::min(c->g.ny - t.j0, std::max(c->g.sponge_north, 3));
+    if (ray_level >= 0 && ray_level < c->g.nz) {
+        t.lev = ray_level;
+        t.ray = c->raycoef;
+    }
+    t.vt = c->vt;
+    t.acc = c->accrain;
+    return DISPATCH(c, kessler_t, c, 1, c->g.nx - 1, 1, c->g.ny - 1, t);
+}
 int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level) { return DISPATCH(c, rayleigh_t, c, il, ih, jl, jh, level); }
 
 int launch_convert(Ctx *c, void *dst, const double *src, size_t n, bool to_device_type) {
