@@ -402,15 +402,12 @@ __global__ void k_upper_lower(Geo g, FieldList fl, int nfill) {
 // The step-end boundary update of theta and moisture in the columns it changes only at the lid, done
 // by Kessler's threads of levels 1 and NZ-2 (the lid takes their values), with theta's Rayleigh damping
 // from level `lev` up (after the lid, as the reference orders them): columns i in [i0, i1), j in [j0, j1).
-// The thread of level 1 also adds the column's precip_rate (it reads level 0 before the lid overwrites
-// it), and k_precip skips those columns.
+// qr's bottom lid waits for k_precip, which reads level 0 before it writes the lid.
 struct KessTail {
     int on = 0;
     int i0 = 0, i1 = 0, j0 = 0, j1 = 0;
     int lev = 1 << 30;
     const double *ray = nullptr;
-    const void *vt = nullptr;
-    void *acc = nullptr;
     __device__ bool in(int i, int j) const { return on && i >= i0 && i < i1 && j >= j0 && j < j1; }
 };
 template <typename T>
@@ -467,19 +464,12 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     if (tail.in(i, j)) {
         auto damp = [&](int kk, T v) { return kk >= tail.lev ? v - T(tail.ray[kk]) * v : v; };
         const int kl = k == 1 ? 0 : (k == g.NZ - 2 ? g.NZ - 1 : -1);  // the lid level this level feeds
-        if (k == 1) {  // precip_rate (microphysics.cpp:143-160) with level 0 before the lid
-            const T *VT = (const T *)tail.vt;
-            const long long p0 = p - 1;
-            T *acc = (T *)tail.acc;
-            const T q0 = QRP[p0];
-            acc[(long long)i * g.NY + j] += T(c.rhow[1]) * T(0.5) * (VT[p0] + VT[p0 + 1]) * T(0.5) * (q0 + qr) * T(g.dt);
-        }
         if (kl >= 0) {
             const long long pl = p + (kl - k);
             THP[pl] = damp(kl, th);
             QVP[pl] = qv;
             QCP[pl] = qc;
-            QRP[pl] = qr;
+            if (kl != 0) QRP[pl] = qr;
         }
         th = damp(k, th);
     }
@@ -491,13 +481,14 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
 
 // precip_rate, microphysics.cpp:143-160
 template <typename T>
-__global__ void k_precip(Geo g, Cols c, const T *__restrict__ VT, const T *__restrict__ QRP, T *__restrict__ acc, int il, int ih,
+__global__ void k_precip(Geo g, Cols c, const T *__restrict__ VT, T *__restrict__ QRP, T *__restrict__ acc, int il, int ih,
                          int jl, int jh, KessTail tail) {
     int j = blockIdx.x * blockDim.x + threadIdx.x + jl;
     int i = blockIdx.y + il;
-    if (j >= jh || i >= ih || tail.in(i, j)) return;
+    if (j >= jh || i >= ih) return;
     long long p = IX(i, j, 0);
     acc[(long long)i * g.NY + j] += T(c.rhow[1]) * T(0.5) * (VT[p] + VT[p + 1]) * T(0.5) * (QRP[p] + QRP[p + 1]) * T(g.dt);
+    if (tail.in(i, j)) QRP[p] = QRP[p + 1];  // the step-end lid of qr, after the rate read level 0
 }
 
 // rayleigh_damping, damping.cpp:1148-1177; coefficients [0..NZ) for u/v/theta, [NZ..2NZ) for w
@@ -723,8 +714,6 @@ int launch_kessler_step_end(Ctx *c, int ray_level) {
         t.lev = ray_level;
         t.ray = c->raycoef;
     }
-    t.vt = c->vt;
-    t.acc = c->accrain;
     return DISPATCH(c, kessler_t, c, 1, c->g.nx - 1, 1, c->g.ny - 1, t);
 }
 int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level) { return DISPATCH(c, rayleigh_t, c, il, ih, jl, jh, level); }
