@@ -499,7 +499,7 @@ inline bool pow2_plan48(int n, Radix &rad, bool radix8) {
 }
 
 template <typename T, int MAXB, bool POW2 = false>
-__device__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, int sb, const T *twl, int sign) {
+__device__ __forceinline__ void lds_fft_ip(T *re, T *im, const Radix &rad, int nbatch, int sn, int sb, const T *twl, int sign) {
     const int n = rad.n;
     const int nthr = blockDim.x;
     int bq[MAXB], jq[MAXB];
