@@ -1037,6 +1037,91 @@ __global__ void __launch_bounds__(kThreadsX) k_irfft_x_out(Geo g, const T *__res
     irfft_x_out_block<T>(g, spec, PI, rad, tw, nk2, kc, blockIdx.x, blockIdx.y, smem, half);
 }
 
+// The same after the half-slab j pass, one block per row pair (j', jhalf + j') and chunk of kc levels:
+// it reads v0 (row j') and v1 (row jhalf + j') once for both rows (v0 + v1 and v0 - v1), and transforms
+// the two rows as one batch of kc level pairs.  Values as irfft_x_out_block's, bit for bit.
+template <typename T>
+__device__ __forceinline__ void irfft_x_out_pair_block(const Geo &g, const T *__restrict__ spec, T *__restrict__ PI, const Radix &rad,
+                                                       const double *tw, int nk2, int kc, int jp, int chunk, unsigned char *smem,
+                                                       int jhalf) {
+    const int n = rad.n, P = kc / 2, B = 2 * P, half = n / 2;  // B: batch of both rows' level pairs
+    T *re = (T *)smem, *im = re + n * B, *twl = im + n * B;
+    const int kk0 = chunk * kc;
+    load_twiddles(twl, tw, n);
+#pragma unroll 2
+    for (int idx = threadIdx.x; idx < n * P; idx += blockDim.x) {
+        int pp = idx % P, m = idx / P;
+        int ka = kk0 + 2 * pp, kb = ka + 1;
+        int ms = m <= half ? m : n - m;
+        const long long o0 = ((long long)ms * g.NY + jp) * nk2, o1 = ((long long)ms * g.NY + jhalf + jp) * nk2;
+        T e[4] = {T(0), T(0), T(0), T(0)}, o[4] = {T(0), T(0), T(0), T(0)};  // a re, a im, b re, b im
+        if (ka < nk2) {
+            e[0] = spec[(o0 + ka) * 2];
+            e[1] = spec[(o0 + ka) * 2 + 1];
+            o[0] = spec[(o1 + ka) * 2];
+            o[1] = spec[(o1 + ka) * 2 + 1];
+        }
+        if (kb < nk2) {
+            e[2] = spec[(o0 + kb) * 2];
+            e[3] = spec[(o0 + kb) * 2 + 1];
+            o[2] = spec[(o1 + kb) * 2];
+            o[3] = spec[(o1 + kb) * 2 + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const T sg = r == 0 ? T(1) : T(-1);
+            T ar = T(0), ai = T(0), br = T(0), bi = T(0);
+            if (ka < nk2) {
+                ar = e[0] + sg * o[0];
+                ai = e[1] + sg * o[1];
+            }
+            if (kb < nk2) {
+                br = e[2] + sg * o[2];
+                bi = e[3] + sg * o[3];
+            }
+            if (ms == 0 || ms == half) {  // self-conjugate wavenumbers: only the real part is kept
+                ai = T(0);
+                bi = T(0);
+            }
+            if (m > half) {  // conjugate mirror
+                ai = -ai;
+                bi = -bi;
+            }
+            re[m * B + r * P + pp] = ar - bi;
+            im[m * B + r * P + pp] = ai + br;
+        }
+    }
+    __syncthreads();
+    lds_fft_ip<T, kMaxBX>(re, im, rad, B, B, 1, twl, +1);
+    const int shift = g.NX - 6;
+    for (int idx = threadIdx.x; idx < 2 * n * kc; idx += blockDim.x) {
+        const int r = idx / (n * kc), rem = idx - r * (n * kc);
+        const int b = rem % kc, i = rem / kc;
+        const int kk = kk0 + b;
+        if (kk >= nk2) continue;
+        const int q = r * P + (b >> 1);
+        T v = (b & 1) ? im[i * B + q] : re[i * B + q];
+        const int j = r == 0 ? jp : jhalf + jp;
+        int ia = i + 3, k = kk + 1;
+        int ks[3] = {k, -1, -1};
+        if (kk == 0) ks[1] = 0;
+        if (kk == nk2 - 1) ks[2] = g.NZ - 1;
+        for (int t = 0; t < 3; t++) {
+            if (ks[t] < 0) continue;
+            PI[IX(ia, j, ks[t])] = v;
+            if (ia >= shift && ia < shift + 3) PI[IX(ia - shift, j, ks[t])] = v;
+            if (ia >= 3 && ia < 6) PI[IX(ia + shift, j, ks[t])] = v;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreadsX) k_irfft_x_out_pair(Geo g, const T *__restrict__ spec, T *__restrict__ PI, Radix rad,
+                                                               const double *tw, int nk2, int kc, int jhalf) {
+    extern __shared__ unsigned char smem[];
+    irfft_x_out_pair_block<T>(g, spec, PI, rad, tw, nk2, kc, blockIdx.x, blockIdx.y, smem, jhalf);
+}
+
 // Q2 on half slabs (even nj, L = nj/2): the j transform's first radix-2 split (decimation in frequency)
 // is done while loading, so block (kx, h) holds only a_h(j') = (x(j') + (-1)^h x(j'+L)) W^(h j'),
 // W = exp(-2 pi i / nj), j' < L, whose length-L transform is X(2m + h).  It solves the columns
@@ -1189,6 +1274,25 @@ bool factorize(int n, Radix &rad) {
 }
 
 int level_chunk(int n, size_t elt, int nk2);
+size_t x_pass_lds(int ni, int kc, size_t elt);
+
+// the inverse i pass into `out`: after the half-slab j pass (half = nj/2) one block per row pair and
+// chunk of an even number of levels, at most kc/2 (same LDS and butterflies per thread as kc levels of
+// one row); otherwise one block per row and chunk of kc levels
+template <typename T>
+void launch_irfft_x_out(Ctx *c, const T *spec, T *out, const Radix &rx, int kc, int half) {
+    const PressurePlan &pp = c->pp;
+    const Geo &g = c->geo;
+    const size_t shx = x_pass_lds(pp.ni, kc, sizeof(T));
+    const int kcp = (kc / 2) & ~1;
+    if (half && kcp >= 2) {
+        hipLaunchKernelGGL(k_irfft_x_out_pair<T>, dim3(half, (pp.nk2 + kcp - 1) / kcp), dim3(kThreadsX), shx, c->stream, g, spec, out, rx,
+                           pp.twx, pp.nk2, kcp, half);
+        return;
+    }
+    hipLaunchKernelGGL(k_irfft_x_out<T>, dim3(g.NY, (pp.nk2 + kc - 1) / kc), dim3(kThreadsX), shx, c->stream, g, spec, out, rx, pp.twx,
+                       pp.nk2, kc, half);
+}
 
 // one j transform of the three-kernel j pass over nrows kx rows: the power-of-two kernel when nj is a
 // power of two and its plan fits, else the mixed-radix one
@@ -1317,8 +1421,7 @@ int solve_t(Ctx *c, double step, bool halos) {
         }
         {
             Timed _t(c, "p5_irfft_x_out", pts);
-            hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)(half ? (T *)pp.spec2 : spec), (T *)c->pi,
-                               rx, pp.twx, pp.nk2, kc, half);
+            launch_irfft_x_out<T>(c, half ? (const T *)pp.spec2 : spec, (T *)c->pi, rx, kc, half);
         }
         DYN_LAUNCH_CHECK();
         return DYN_OK;
@@ -1484,8 +1587,7 @@ int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T
     }
     {
         Timed _t(c, "adj_irfft_x", pts);
-        hipLaunchKernelGGL(k_irfft_x_out<T>, gx, dim3(kThreadsX), shx, c->stream, g, (const T *)(half ? (T *)pp.spec2 : spec), r2, rx, pp.twx,
-                           pp.nk2, kc, half);
+        launch_irfft_x_out<T>(c, half ? (const T *)pp.spec2 : spec, r2, rx, kc, half);
     }
     DYN_LAUNCH_CHECK();
     {
