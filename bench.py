@@ -227,8 +227,8 @@ def main():
                          "1: the whole step on one stream (per-kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would shard); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the dynamical core here has no multi-GPU path; run with --gpus 1",
+              file=sys.stderr)
         return 2
 
     from model_amd import configs

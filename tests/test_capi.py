@@ -63,3 +63,14 @@ def test_oracle_is_not_reachable_from_the_product():
                 assert "oracle" not in text.lower() or fn == "__init__.py" and "oracle" not in text, os.path.join(dirpath, fn)
     ldd = subprocess.run(["ldd", core.LIB_PATH], capture_output=True, text=True).stdout
     assert "oracle" not in ldd
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N above 1 exits with a message (there is no multi-GPU path)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
