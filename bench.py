@@ -222,9 +222,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint measurements")
-    ap.add_argument("--stage-streams", type=int, choices=[1, 2], default=2,
-                    help="2: a stage's scalar half runs on a second stream beside the pressure solve (default); "
-                         "1: the whole step on one stream (per-kernel profiles)")
+    ap.add_argument("--stage-streams", type=int, choices=[0, 1, 2, 3], default=0,
+                    help="0: the library's default; 2: a stage's scalar half runs on a second stream beside the "
+                         "pressure solve, forked after the momentum kernel; 3: forked after the solve's first "
+                         "pass; 1: the whole step on one stream (per-kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the dynamical core here has no multi-GPU path; run with --gpus 1",
@@ -243,7 +244,9 @@ def main():
     handles = {}
     for name, prec in forms:
         g, ms = time_form(case, prec, args.steps, args.warmup, args.stage_streams)
-        res[name] = {"ms_per_step": round(ms, 4), "value": round(npts / (ms * 1e-3), 1)}
+        from model_amd.core import DYN_TUNE_STAGE_STREAMS
+        res[name] = {"ms_per_step": round(ms, 4), "value": round(npts / (ms * 1e-3), 1),
+                     "stage_streams": g.tune_get(DYN_TUNE_STAGE_STREAMS)}
         handles[name] = g
         log(f"bench: {name} {ms:.3f} ms/step, {res[name]['value']:.4g} gridpoint-steps/s")
     main_form = forms[0][0]
@@ -267,7 +270,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "moist Kessler supercell, 256x256x64 (NX=262 incl. halo), dx=1 km, dz=250 m, dt=2 s",
                    "nx": 256, "ny": 256, "nz": 64, "form": main_form, "step": "hipGraph replay of s_integrate_rk3",
-                   "stage_streams": args.stage_streams},
+                   "stage_streams": res[main_form]["stage_streams"]},
         "forms": res,
         "fp32_over_fp64": round(res["fp32"]["value"] / res["fp64"]["value"], 3) if "fp64" in res and "fp32" in res else None,
         "adjoint_forward_ratio": None,

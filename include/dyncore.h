@@ -135,8 +135,10 @@ enum {
     DYN_TUNE_SCALAR_TI = 3,   /* columns one scalar block marches along i: 2 .. 512 */
     DYN_TUNE_PRESSURE_JPASS = 4, /* pressure j pass: 1 one LDS-resident kernel per kx (when the slab fits), 2 three
                                     kernels, 3 two LDS-resident kernels per kx on half slabs (even NY, when they fit) */
-    DYN_TUNE_STAGE_STREAMS = 5,  /* fused step: 2 runs a stage's scalar half on a second stream beside the
-                                    pressure solve, 1 runs the whole stage on the context's stream */
+    DYN_TUNE_STAGE_STREAMS = 5,  /* fused step: 1 runs the whole stage on the context's stream; 2 runs a
+                                    stage's scalar half on a second stream beside the pressure solve, forked
+                                    after the momentum kernel; 3 the same, forked after the solve's first
+                                    pass (the default where the j pass runs on half slabs, else 2) */
     DYN_TUNE_COUNT = 6
 };
 int dyn_tune_set(dyn_ctx *ctx, int param, int value);

@@ -681,6 +681,33 @@ static int reference_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nu
     return DYN_OK;
 }
 
+// The scalar half of a fused stage: theta and moisture advection, then zero_moisture outside the
+// advected box and the scalars' boundary update.  With fork set it runs on the context's second stream,
+// behind an event that orders it after everything the context's stream holds so far.  A failed fork
+// is returned; a failed launch is kept in rc, so that the caller still joins the second stream.
+struct ScalarHalf {
+    double step;
+    bool fork;
+    int rc;
+};
+
+static int launch_scalar_half(Ctx *c, void *arg) {
+    ScalarHalf &h = *(ScalarHalf *)arg;
+    const int NX = c->g.nx, NY = c->g.ny;
+    const hipStream_t own = c->stream;
+    if (h.fork) {
+        if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess || hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) {
+            set_error("fused step: fork onto the second stream failed");
+            return DYN_E_HIP;
+        }
+        c->stream = c->side;
+    }
+    h.rc = launch_fused_advect(c, h.step, 3, NX - 3, 3, NY - 3, kStageScalars);
+    if (!h.rc) h.rc = launch_post_stage(c, h.step, true, true, true, false);
+    c->stream = own;
+    return DYN_OK;
+}
+
 // The same step with the stage advection fused into one kernel (advect_uvw_velocity, advect_theta,
 // advect_microphysics_cell and zero_moisture's interior in one pass).  advect_theta and the moisture
 // advection read only the current-stage fields, which the pressure solve and the gradient force never
@@ -703,35 +730,24 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     do {                           \
         if ((rc = (x))) return rc; \
     } while (0)
-    // The scalar half of a stage (theta and moisture advection, their boundary update) reads the
-    // current buffers only and writes buffers nothing else in the stage reads, so it runs on the second
-    // stream beside the pressure solve, which needs only the u, v, w forecasts of the momentum half.
-    // Started after the momentum kernel (the head of the stage's critical path), its blocks fill the
-    // CUs the latency-bound transforms leave idle.
-    const bool two = fused_shape(c, DYN_TUNE_STAGE_STREAMS) == 2;
-    auto on_side = [&](auto &&launch) {
-        if (!two) return launch();
-        hipStream_t own = c->stream;
-        c->stream = c->side;
-        const int r = launch();
-        c->stream = own;
-        return r;
-    };
+    // The scalar half of a stage reads the current buffers only and writes buffers nothing else in the
+    // stage reads, so it runs on the second stream beside the pressure solve, which needs only the u,
+    // v, w forecasts of the momentum half.  Forked after the momentum kernel (the head of the stage's
+    // critical path), its blocks fill the CUs the latency-bound transforms leave idle.  Where the j pass
+    // runs on half slabs, whose blocks each take most of a CU's LDS, it is forked after the solve's
+    // first pass instead (2.08 -> 1.99 ms per FP64 step at 256x256x64; elsewhere neutral or slower).
+    const int streams = fused_shape(c, DYN_TUNE_STAGE_STREAMS);
+    const bool two = streams >= 2, late = streams == 3;
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
         if (hook) RC(hook(c, s, arg));
         if (s == 1 && shared) RC(device_copy_shell(c, DYN_U, flast));
         RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageMomentum));
-        if (two && (hipEventRecord(c->ev_fork, c->stream) != hipSuccess || hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess)) {
-            set_error("fused step: fork onto the second stream failed");
-            return DYN_E_HIP;
-        }
-        int rs = on_side([&] { return launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageScalars); });
-        // zero_moisture outside the advected box and the scalars' boundary update
-        if (!rs) rs = on_side([&] { return launch_post_stage(c, step, true, true, true, false); });
+        ScalarHalf half{step, two, DYN_OK};
+        if (!late) RC(launch_scalar_half(c, &half));
         // the halo columns periodic_uvw_boundaries writes are read only by the divergence (which reads
         // UP(3) for UP(NX-3)); the boundary update overwrites them before anything else reads them
-        int rm = launch_solve_pressure(c, step, false);
+        int rm = launch_solve_pressure(c, step, false, late ? launch_scalar_half : nullptr, &half);
         // pressure gradient and the boundary update of u, v, w
         if (!rm) rm = launch_post_stage(c, step, true, false, false, true);
         // the join comes first, also after a failed launch, so no work is left on the second stream
@@ -739,7 +755,7 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
             set_error("fused step: join of the second stream failed");
             return DYN_E_HIP;
         }
-        RC(rs);
+        RC(half.rc);
         RC(rm);
         // advance_inner / microphysics_advance_inner; from shared roles the forecast role moves to the
         // spare buffer

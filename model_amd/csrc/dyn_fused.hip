@@ -1172,7 +1172,7 @@ bool fused_shape_ok(const Ctx *c, int param, int value) {
     }
     if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
     if (param == DYN_TUNE_PRESSURE_JPASS) return value == 2 || (value == 1 && c->pp.fused_y) || (value == 3 && c->pp.half_y);
-    if (param == DYN_TUNE_STAGE_STREAMS) return value == 1 || value == 2;
+    if (param == DYN_TUNE_STAGE_STREAMS) return value >= 1 && value <= 3;
     return false;
 }
 
@@ -1196,7 +1196,9 @@ int fused_shape(const Ctx *c, int param) {
     case DYN_TUNE_MOMENTUM_TI:
     case DYN_TUNE_SCALAR_TI: return fused_waves(c) == 2 ? 32 : 16;  // 512x512x80: 32 measured 3 % faster
     case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : (c->pp.half_y ? 3 : 2);
-    case DYN_TUNE_STAGE_STREAMS: return 2;
+    // the scalar half forked after the pressure solve's first pass where the j pass runs on half slabs
+    // (FP64 256x256x64: 4 % faster); after the momentum kernel elsewhere (FP32 there: 7 % faster)
+    case DYN_TUNE_STAGE_STREAMS: return fused_shape(c, DYN_TUNE_PRESSURE_JPASS) == 3 ? 3 : 2;
     default: return DYN_E_ARG;
     }
 }

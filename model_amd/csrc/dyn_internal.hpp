@@ -209,7 +209,10 @@ int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level);
 int rayleigh_table(Ctx *c, int level);
 int pressure_init(Ctx *c);
 int pressure_free(Ctx *c);
-int launch_solve_pressure(Ctx *c, double step, bool halos = true);
+// after_i (optional) is called with arg once the first pass (divergence and forward i transform) is
+// queued; a nonzero return ends the solve with that code
+using PressureHook = int (*)(Ctx *c, void *arg);
+int launch_solve_pressure(Ctx *c, double step, bool halos = true, PressureHook after_i = nullptr, void *arg = nullptr);
 int launch_pressure_adjoint(Ctx *c, double step, void *pihat, void *uh, void *vh, void *wh, void *r1, void *r2);
 int launch_convert(Ctx *c, void *dst, const double *src, size_t n, bool to_device_type);
 void adjoint_release(Ctx *c);

@@ -1357,7 +1357,7 @@ int x_level_chunk(const PressurePlan &pp, const Geo &g, const Radix &rx, size_t 
 }
 
 template <typename T>
-int solve_t(Ctx *c, double step, bool halos) {
+int solve_t(Ctx *c, double step, bool halos, PressureHook after_i, void *arg) {
     PressurePlan &pp = c->pp;
     const Geo &g = c->geo;
     Radix rx{}, ry{};
@@ -1384,6 +1384,10 @@ int solve_t(Ctx *c, double step, bool halos) {
                                (const T *)nullptr);
         }
         DYN_LAUNCH_CHECK();
+        if (after_i) {
+            const int rc = after_i(c, arg);
+            if (rc) return rc;
+        }
         const int pad = 1;
         const size_t shy = y_pass_lds(pp.nj, pp.nk2, sizeof(T));
         const int jpass = fused_shape(c, DYN_TUNE_PRESSURE_JPASS);
@@ -1436,6 +1440,10 @@ int solve_t(Ctx *c, double step, bool halos) {
                            (const T *)c->prog[DYN_V][2], (const T *)c->prog[DYN_W][2], spec, rx, pp.twx, pp.nk2, kcx, step);
     }
     DYN_LAUNCH_CHECK();
+    if (after_i) {
+        const int rc = after_i(c, arg);
+        if (rc) return rc;
+    }
     {
         Timed _t(c, "p2_fft_y", pts);
         hipLaunchKernelGGL(k_fft_y<T>, gy, dim3(kThreads), shy, c->stream, spec, pp.nj, pp.nk2, ry, pp.twy, kcy, -1);
@@ -1715,8 +1723,8 @@ int pressure_free(Ctx *c) {
     return DYN_OK;
 }
 
-int launch_solve_pressure(Ctx *c, double step, bool halos) {
-    return c->prec == DYN_FP32 ? solve_t<float>(c, step, halos) : solve_t<double>(c, step, halos);
+int launch_solve_pressure(Ctx *c, double step, bool halos, PressureHook after_i, void *arg) {
+    return c->prec == DYN_FP32 ? solve_t<float>(c, step, halos, after_i, arg) : solve_t<double>(c, step, halos, after_i, arg);
 }
 
 

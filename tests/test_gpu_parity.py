@@ -385,14 +385,17 @@ def test_nz65_takes_the_per_operator_step():
 
 @pytest.mark.parametrize("precision,moist", [(DYN_FP32, 1), (DYN_FP32, 0), (0, 1)], ids=["fp32-moist", "fp32-dry", "fp64-moist"])
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
-def test_stage_on_two_streams_bitwise(precision, moist, graph):
-    """The scalar half of a stage on the second stream, beside the pressure solve, gives the bits of the
-    whole stage on one stream (all buffer roles and the fall speed)."""
+@pytest.mark.parametrize("fork", [2, 3], ids=["fork-after-momentum", "fork-after-i-pass"])
+def test_stage_on_two_streams_bitwise(precision, moist, graph, fork):
+    """The scalar half of a stage on the second stream, beside the pressure solve (forked after the
+    momentum kernel or after the solve's first pass), gives the bits of the whole stage on one stream
+    (all buffer roles and the fall speed)."""
     from model_amd import core
     c = configs.supercell(nx=40, ny=36, nz=30) if moist else configs.vortex_in_shear(nx=40, ny=36, nz=30)
     a, b = DynCore(c, precision=precision), DynCore(c, precision=precision)
     a.tune_set(core.DYN_TUNE_STAGE_STREAMS, 1)
     assert b.tune_get(core.DYN_TUNE_STAGE_STREAMS) == 2
+    b.tune_set(core.DYN_TUNE_STAGE_STREAMS, fork)
     for g in (a, b):
         (g.step if graph else g.step_fused)(5)
         g.sync()
@@ -416,6 +419,8 @@ def test_j_pass_forms_against_oracle(jpass, graph):
     g = DynCore(c)
     g.tune_set(core.DYN_TUNE_PRESSURE_JPASS, jpass)
     assert g.tune_get(core.DYN_TUNE_PRESSURE_JPASS) == jpass
+    # the fused step forks its scalar half after the solve's first pass where the j pass runs on half slabs
+    assert g.tune_get(core.DYN_TUNE_STAGE_STREAMS) == (3 if jpass == 3 else 2)
     o.step(8)
     (g.step if graph else g.step_reference)(8)
     g.sync()
@@ -423,6 +428,22 @@ def test_j_pass_forms_against_oracle(jpass, graph):
         e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
         assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
     assert rel_err(g.get_field(ids.PI, 0), o.get_field(ids.PI, 0)) <= 1e-11
+
+
+def test_odd_transform_length_fork_after_i_pass_bitwise():
+    """The odd-length solve (full complex spectrum) forks the scalar half after its first pass too: the
+    step then gives the bits of the whole stage on one stream, every role of every prognostic."""
+    from model_amd import core
+    c = configs.small_case(nx=15, ny=18, nz=12, microphysics=1, amplitude=0.3)
+    a, b = DynCore(c, precision=DYN_FP32), DynCore(c, precision=DYN_FP32)
+    a.tune_set(core.DYN_TUNE_STAGE_STREAMS, 1)
+    b.tune_set(core.DYN_TUNE_STAGE_STREAMS, 3)
+    for g in (a, b):
+        g.step(5)
+        g.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR):
+        for role in (ids.ROLE_M, CUR, ids.ROLE_P):
+            assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), (ids.FIELD_NAMES[f], role)
 
 
 @pytest.mark.parametrize("graph", [False, True], ids=["per-operator", "graph"])
