@@ -1194,13 +1194,14 @@ int fused_shape(const Ctx *c, int param) {
         }
         return tj;
     case DYN_TUNE_MOMENTUM_TI: return fused_waves(c) == 2 ? 32 : 16;  // 512x512x80: 32 measured 3 % faster
-    // The FP32 scalar kernel beside the pressure solve (fork after the momentum kernel): 24 columns, 2-4 %
-    // faster per step than 16 at 256x256x64 (moist) and 192x192x40 (dry).  Its 352 blocks at 256x256
-    // leave most CUs one block, beside which a j-pass block still fits; alone the kernel is slower
-    // that way (165 against 120 us), so the one-stream step and FP64 keep 16.
+    // The scalar kernel beside the pressure solve (fork after the momentum kernel): 24 columns, about 2 %
+    // faster per step than 16 (FP32 at 256x256x64 moist and 192x192x40 dry, FP64 at 192x192x40 dry).
+    // Its 352 blocks at 256x256 leave most CUs one block, beside which a j-pass block still fits; alone
+    // the kernel is slower that way (165 against 120 us), so the one-stream step keeps 16, and so does
+    // the later fork, where 16 measured best.
     case DYN_TUNE_SCALAR_TI:
         if (fused_waves(c) == 2) return 32;
-        return f32 && fused_shape(c, DYN_TUNE_STAGE_STREAMS) == 2 ? 24 : 16;
+        return fused_shape(c, DYN_TUNE_STAGE_STREAMS) == 2 ? 24 : 16;
     case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : (c->pp.half_y ? 3 : 2);
     // the scalar half forked after the pressure solve's first pass where the j pass runs on half slabs
     // (FP64 256x256x64: 4 % faster); after the momentum kernel elsewhere (FP32 there: 7 % faster)
