@@ -222,10 +222,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint measurements")
-    ap.add_argument("--stage-streams", type=int, choices=[0, 1, 2, 3], default=0,
+    ap.add_argument("--stage-streams", type=int, choices=[0, 1, 2, 3, 4], default=0,
                     help="0: the library's default; 2: a stage's scalar half runs on a second stream beside the "
                          "pressure solve, forked after the momentum kernel; 3: forked after the solve's first "
-                         "pass; 1: the whole step on one stream (per-kernel profiles)")
+                         "pass; 4: forked after the momentum kernel, queued after the first pass; 1: the whole "
+                         "step on one stream (per-kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the dynamical core here has no multi-GPU path; run with --gpus 1",

@@ -138,7 +138,9 @@ enum {
     DYN_TUNE_STAGE_STREAMS = 5,  /* fused step: 1 runs the whole stage on the context's stream; 2 runs a
                                     stage's scalar half on a second stream beside the pressure solve, forked
                                     after the momentum kernel; 3 the same, forked after the solve's first
-                                    pass (the default where the j pass runs on half slabs, else 2) */
+                                    pass; 4 forked after the momentum kernel, its launches queued after the
+                                    solve's first pass.  Default: 3 where the j pass runs on half slabs,
+                                    else 2 (FP32) or 4 (FP64) */
     DYN_TUNE_COUNT = 6
 };
 int dyn_tune_set(dyn_ctx *ctx, int param, int value);

@@ -683,11 +683,13 @@ static int reference_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nu
 
 // The scalar half of a fused stage: theta and moisture advection, then zero_moisture outside the
 // advected box and the scalars' boundary update.  With fork set it runs on the context's second stream,
-// behind an event that orders it after everything the context's stream holds so far.  A failed fork
-// is returned; a failed launch is kept in rc, so that the caller still joins the second stream.
+// behind an event that orders it after everything the context's stream holds so far, or, with
+// recorded set, behind the fork event the caller recorded earlier.  A failed fork is returned; a
+// failed launch is kept in rc, so that the caller still joins the second stream.
 struct ScalarHalf {
     double step;
     bool fork;
+    bool recorded;
     int rc;
 };
 
@@ -696,7 +698,8 @@ static int launch_scalar_half(Ctx *c, void *arg) {
     const int NX = c->g.nx, NY = c->g.ny;
     const hipStream_t own = c->stream;
     if (h.fork) {
-        if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess || hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) {
+        if ((!h.recorded && hipEventRecord(c->ev_fork, c->stream) != hipSuccess) ||
+            hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) {
             set_error("fused step: fork onto the second stream failed");
             return DYN_E_HIP;
         }
@@ -736,14 +739,20 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     // critical path), its blocks fill the CUs the latency-bound transforms leave idle.  Where the j pass
     // runs on half slabs, whose blocks each take most of a CU's LDS, it is forked after the solve's
     // first pass instead (2.08 -> 1.99 ms per FP64 step at 256x256x64; elsewhere neutral or slower).
+    // Value 4 keeps the fork after the momentum kernel but queues (and captures) the scalar launches
+    // after the solve's first pass, which measured faster for FP64 on the other grids.
     const int streams = fused_shape(c, DYN_TUNE_STAGE_STREAMS);
-    const bool two = streams >= 2, late = streams == 3;
+    const bool two = streams >= 2, late = streams >= 3;
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
         if (hook) RC(hook(c, s, arg));
         if (s == 1 && shared) RC(device_copy_shell(c, DYN_U, flast));
         RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageMomentum));
-        ScalarHalf half{step, two, DYN_OK};
+        ScalarHalf half{step, two, streams == 4, DYN_OK};
+        if (half.recorded && hipEventRecord(c->ev_fork, c->stream) != hipSuccess) {
+            set_error("fused step: fork onto the second stream failed");
+            return DYN_E_HIP;
+        }
         if (!late) RC(launch_scalar_half(c, &half));
         // the halo columns periodic_uvw_boundaries writes are read only by the divergence (which reads
         // UP(3) for UP(NX-3)); the boundary update overwrites them before anything else reads them

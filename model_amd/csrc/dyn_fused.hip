@@ -1172,7 +1172,7 @@ bool fused_shape_ok(const Ctx *c, int param, int value) {
     }
     if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
     if (param == DYN_TUNE_PRESSURE_JPASS) return value == 2 || (value == 1 && c->pp.fused_y) || (value == 3 && c->pp.half_y);
-    if (param == DYN_TUNE_STAGE_STREAMS) return value >= 1 && value <= 3;
+    if (param == DYN_TUNE_STAGE_STREAMS) return value >= 1 && value <= 4;
     return false;
 }
 
@@ -1201,11 +1201,15 @@ int fused_shape(const Ctx *c, int param) {
     // the later fork, where 16 measured best.
     case DYN_TUNE_SCALAR_TI:
         if (fused_waves(c) == 2) return 32;
-        return fused_shape(c, DYN_TUNE_STAGE_STREAMS) == 2 ? 24 : 16;
+        {
+            const int streams = fused_shape(c, DYN_TUNE_STAGE_STREAMS);
+            return streams == 2 || streams == 4 ? 24 : 16;
+        }
     case DYN_TUNE_PRESSURE_JPASS: return c->pp.fused_y ? 1 : (c->pp.half_y ? 3 : 2);
     // the scalar half forked after the pressure solve's first pass where the j pass runs on half slabs
-    // (FP64 256x256x64: 4 % faster); after the momentum kernel elsewhere (FP32 there: 7 % faster)
-    case DYN_TUNE_STAGE_STREAMS: return fused_shape(c, DYN_TUNE_PRESSURE_JPASS) == 3 ? 3 : 2;
+    // (FP64 256x256x64: 4 % faster); after the momentum kernel elsewhere (FP32 there: 7 % faster), in
+    // FP64 with its launches queued after the first pass (192x192x40: 5 % faster)
+    case DYN_TUNE_STAGE_STREAMS: return fused_shape(c, DYN_TUNE_PRESSURE_JPASS) == 3 ? 3 : (f32 ? 2 : 4);
     default: return DYN_E_ARG;
     }
 }

@@ -385,16 +385,17 @@ def test_nz65_takes_the_per_operator_step():
 
 @pytest.mark.parametrize("precision,moist", [(DYN_FP32, 1), (DYN_FP32, 0), (0, 1)], ids=["fp32-moist", "fp32-dry", "fp64-moist"])
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
-@pytest.mark.parametrize("fork", [2, 3], ids=["fork-after-momentum", "fork-after-i-pass"])
+@pytest.mark.parametrize("fork", [2, 3, 4], ids=["fork-after-momentum", "fork-after-i-pass", "fork-after-momentum-queued-late"])
 def test_stage_on_two_streams_bitwise(precision, moist, graph, fork):
     """The scalar half of a stage on the second stream, beside the pressure solve (forked after the
-    momentum kernel or after the solve's first pass), gives the bits of the whole stage on one stream
-    (all buffer roles and the fall speed)."""
+    momentum kernel, after the solve's first pass, or after the momentum kernel with its launches queued
+    after the first pass), gives the bits of the whole stage on one stream (all buffer roles and the
+    fall speed)."""
     from model_amd import core
     c = configs.supercell(nx=40, ny=36, nz=30) if moist else configs.vortex_in_shear(nx=40, ny=36, nz=30)
     a, b = DynCore(c, precision=precision), DynCore(c, precision=precision)
     a.tune_set(core.DYN_TUNE_STAGE_STREAMS, 1)
-    assert b.tune_get(core.DYN_TUNE_STAGE_STREAMS) == 2
+    assert b.tune_get(core.DYN_TUNE_STAGE_STREAMS) == (2 if precision == DYN_FP32 else 4)
     # the scalar kernel marches 24 columns beside the pressure solve (forked after momentum), 16 alone
     assert a.tune_get(core.DYN_TUNE_SCALAR_TI) == 16
     assert b.tune_get(core.DYN_TUNE_SCALAR_TI) == 24
@@ -422,8 +423,9 @@ def test_j_pass_forms_against_oracle(jpass, graph):
     g = DynCore(c)
     g.tune_set(core.DYN_TUNE_PRESSURE_JPASS, jpass)
     assert g.tune_get(core.DYN_TUNE_PRESSURE_JPASS) == jpass
-    # the fused step forks its scalar half after the solve's first pass where the j pass runs on half slabs
-    assert g.tune_get(core.DYN_TUNE_STAGE_STREAMS) == (3 if jpass == 3 else 2)
+    # the fused step forks its scalar half after the solve's first pass where the j pass runs on half
+    # slabs; in FP64 elsewhere after momentum, with the scalar launches queued after the first pass
+    assert g.tune_get(core.DYN_TUNE_STAGE_STREAMS) == (3 if jpass == 3 else 4)
     o.step(8)
     (g.step if graph else g.step_reference)(8)
     g.sync()
