@@ -3,8 +3,8 @@
 Tolerances: the stencil, boundary and pressure-gradient kernels evaluate the reference's expression
 trees without FMA contraction, so they agree with the oracle to the last bit; the pressure solve uses a
 different FFT factorisation (rounding ~1e-15); Kessler uses the device exp/pow (a few ulp).  Whole runs
-are held to the north-star bar: 1e-11 relative (max-abs difference over the field's max-abs value) in
-every prognostic field.
+of the FP64 plain form are held to 1e-11 relative (max-abs difference over the field's max-abs value)
+in every prognostic field.
 """
 import ctypes
 import os
