@@ -146,7 +146,9 @@ int device_copy_fields(Ctx *c, int f0, int f1) {
 // the lid).  The reference's step sequence, run on the CPU with the rest of that buffer poisoned,
 // gives the same bits as without the poison, so copying w whole and the six-column frame of the other
 // fields from the starting state gives every value the reference has.
-int device_copy_shell(Ctx *c, int f0, int f1) {
+// to_spare: copy into the spare buffers (stage 2's output buffers while stage 1 still runs) instead of
+// the forecast role.
+int device_copy_shell(Ctx *c, int f0, int f1, bool to_spare = false) {
     const int NX = c->g.nx, NY = c->g.ny, NZ = c->g.nz;
     const long long words = (long long)(c->npts * c->elt / 4);
     const unsigned wblocks = (unsigned)std::max<long long>(1, std::min<long long>((words / 4 + 255) / 256, 2048));
@@ -154,9 +156,14 @@ int device_copy_shell(Ctx *c, int f0, int f1) {
     FieldCopies whole{}, frame{};
     int nw = 0, nf = 0;
     for (int f = f0; f <= f1; f++) {
+        void *dst = to_spare ? c->spare[f] : c->prog[f][DYN_ROLE_P];
+        if (!dst) {
+            set_error("fused step: no spare buffer for the stage-2 copy");
+            return DYN_E_STATE;
+        }
         const bool all = f == DYN_W || NX < 12 || NY < 12;  // narrow grids: no interior beyond the frame
         (all ? whole.src[nw] : frame.src[nf]) = c->prog[f][DYN_ROLE_M];
-        (all ? whole.dst[nw++] : frame.dst[nf++]) = c->prog[f][DYN_ROLE_P];
+        (all ? whole.dst[nw++] : frame.dst[nf++]) = dst;
     }
     hipLaunchKernelGGL(k_copy_fields, dim3(wblocks, nw), dim3(256), 0, c->stream, whole, words);
     DYN_LAUNCH_CHECK();
@@ -743,10 +750,14 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
     // after the solve's first pass, which measured faster for FP64 on the other grids.
     const int streams = fused_shape(c, DYN_TUNE_STAGE_STREAMS);
     const bool two = streams >= 2, late = streams >= 3;
+    // With shared roles on two streams, the copy into stage 2's output buffer runs during stage 1, on
+    // the second stream after the scalar half: it reads the step's starting state and writes the spare
+    // buffer, neither of which stage 1 writes or reads, and the second stream's chain is the shorter one.
+    const bool copy_beside = shared && two;
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
         if (hook) RC(hook(c, s, arg));
-        if (s == 1 && shared) RC(device_copy_shell(c, DYN_U, flast));
+        if (s == 1 && shared && !copy_beside) RC(device_copy_shell(c, DYN_U, flast));
         RC(launch_fused_advect(c, step, 3, NX - 3, 3, NY - 3, kStageMomentum));
         ScalarHalf half{step, two, streams == 4, DYN_OK};
         if (half.recorded && hipEventRecord(c->ev_fork, c->stream) != hipSuccess) {
@@ -759,6 +770,12 @@ static int fused_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullpt
         int rm = launch_solve_pressure(c, step, false, late ? launch_scalar_half : nullptr, &half);
         // pressure gradient and the boundary update of u, v, w
         if (!rm) rm = launch_post_stage(c, step, true, false, false, true);
+        if (s == 0 && copy_beside && !rm && !half.rc) {  // the scalar half is on the second stream by now
+            const hipStream_t own = c->stream;
+            c->stream = c->side;
+            rm = device_copy_shell(c, DYN_U, flast, true);
+            c->stream = own;
+        }
         // the join comes first, also after a failed launch, so no work is left on the second stream
         if (two && (hipEventRecord(c->ev_join, c->side) != hipSuccess || hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess)) {
             set_error("fused step: join of the second stream failed");
