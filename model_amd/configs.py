@@ -219,13 +219,15 @@ def vortex_in_shear(nx=192, ny=192, nz=40, dx=20000.0, dz=500.0, dt=60.0, steps=
     fields = _empty_fields(NX, NY, NZ)
     fields["pbar"][:] = cols["pib"][None, None, :]
     x, y, zs = _coords(NX, NY, cols, dx)
-    # basic-state u(y,z): horizontal shear strip (max vorticity zeta0) times a piecewise-linear
-    # vertical profile with shears -1e-3 below and -2e-3 above 5 km
+    # basic-state u(y,z): a horizontal shear strip plus a piecewise-linear vertical profile with shears
+    # -1e-3 below and -2e-3 above 5 km.  The strip is u = -zeta0 W tanh((y - yc)/W): its vorticity
+    # -du/dy peaks at zeta0 on the centre line; W = 150 km gives the width of the reference's 18-21 N
+    # strip (data_initializer.cpp:350, about 330 km) and a jet of +-15 m/s.
     yc = 0.5 * (ny - 1) * dx
-    width = 0.25 * ny * dx
+    width = 150e3
     uy = -zeta0 * width * np.tanh((y - yc) / width)
     uz = np.where(zs < 5000.0, -1e-3 * zs, -1e-3 * 5000.0 - 2e-3 * (zs - 5000.0))
-    fields["ubar"][:] = (uy[:, None] * 0.1 + uz[None, :])[None, :, :]
+    fields["ubar"][:] = (uy[:, None] + uz[None, :])[None, :, :]
     # axisymmetric Gaussian-vorticity vortex: v_t(r) = zeta0 R^2/(2r) (1 - exp(-r^2/R^2)) e^{-z/H}
     xc = 0.5 * (nx - 1) * dx
     X, Y, Z = np.meshgrid(x, y, zs, indexing="ij")

@@ -18,16 +18,23 @@
  *   orc_poisson_rhs            manufactured solution: the discrete operator applied to a random
  *                              periodic field is inverted to 1e-12, with the (0,0) gauge
  *   orc_kessler_point          per-point conservation of qv+qc+qr, the latent-heat balance,
- *                              saturation after adjustment, inactive-branch identities
- *   orc_fall_speed             closed form (kessler.cpp:222-252)
+ *                              saturation after adjustment, inactive-branch identities; each rate
+ *                              term against the published Klemp & Wilhelmson (1978) forms in cgs
+ *                              units with the SI conversion derived in the test, the saturation
+ *                              pressure against IAPWS values, the adjustment as a Newton step
+ *                              (tests/test_kessler_published_forms.py)
+ *   orc_fall_speed             Klemp & Wilhelmson's 36.34 (rho qr)^0.1364 (rho_0/rho)^1/2 in cgs
  *   orc_advect_* , orc_pgf,    closed-form single-term cases: uniform fields, linear profiles,
  *   orc_solve_pressure,        staggered advecting velocities, buoyancy overwrite, ground flux,
  *   orc_apply_bc, orc_rayleigh projection to zero divergence, sponge/periodic/lid ranges, damping
  *                              profile, rest state (tests/test_operators_closed_form.py)
- *   orc_step                   the composition: parity unpinned against the reference's own
- *                              serial build (not buildable here); its operator order, written
- *                              ranges and buffer rotation follow driver_serial.cpp:178-286 and
- *                              solver.cpp:229-263 line by line.
+ *   orc_step                   the composition: a Fourier mode advected by a uniform wind grows
+ *                              by the Wicker-Skamarock RK3 polynomial of the published fifth-order
+ *                              upwind symbol, over one and three steps, to 1e-12, and stages built
+ *                              on the current buffer instead of M fail it
+ *                              (tests/test_rk3_fourier_mode.py).  Not compared with the reference's
+ *                              own serial build (not buildable here); its written ranges and ghost
+ *                              rows follow driver_serial.cpp:178-286 and solver.cpp:229-263.
  */
 #ifndef DYNCORE_ORACLE_H
 #define DYNCORE_ORACLE_H
