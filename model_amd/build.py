@@ -19,7 +19,9 @@ LIB = os.path.join(OUT, "libdyncore.so")
 REFAPI_LIB = os.path.join(OUT, "libdyncore_refapi.so")
 
 HIP_SOURCES = ["dyn_capi.hip", "dyn_stencil.hip", "dyn_pressure.hip", "dyn_fused.hip", "dyn_adjoint.hip", "dyn_ensemble.hip"]
-REFAPI_SOURCES = ["refapi/refapi.cpp", "refapi/driver_serial.cpp"]
+REFAPI_SOURCES = ["refapi/refapi.cpp"]
+# the C++ symbols libdyncore_refapi.so defines: what a maintainer localizes in the reference's objects
+REFAPI_REPLACED = os.path.join(OUT, "libdyncore_refapi.replaced")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DYN_OFFLOAD_ARCH", "gfx950")
@@ -66,6 +68,12 @@ def build(verbose: bool = False) -> str:
         _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", "-o", tmp, *srcs,
               f"-L{OUT}", "-ldyncore", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, REFAPI_LIB)
+        nm = _run(["nm", "-D", "--defined-only", REFAPI_LIB]).stdout.splitlines()
+        syms = sorted({ln.split()[2] for ln in nm if len(ln.split()) == 3 and ln.split()[1] == "T"
+                       and ln.split()[2].startswith("_Z")})
+        with open(REFAPI_REPLACED + ".tmp", "w") as fh:
+            fh.write("\n".join(syms) + "\n")
+        os.replace(REFAPI_REPLACED + ".tmp", REFAPI_REPLACED)
     if verbose:
         print("built", LIB)
     return LIB

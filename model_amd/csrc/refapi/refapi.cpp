@@ -2,7 +2,9 @@
 //
 // Each function keeps the name, signature and argument meaning of the reference function it replaces
 // (cited per function) and forwards to the matching dyn_* call.  Device buffers rotate together with
-// the host pointer swaps the reference performs, so host globals and HBM stay paired by role.
+// the host pointer swaps the reference performs (switch_array, solver.cpp:229-235), so host globals and
+// HBM stay paired by role.  The host globals are the reference's own (initializer.cpp defines them);
+// this library only references them.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -158,20 +160,10 @@ void advect_scalar(double, int, int, int, int, double *, double *, double *, dou
     unsupported("advect_scalar");
 }
 
-// ---- fluxes.h: per-slab scratch and sign cells are formed inside the stage kernels ----------------
-void initialize_flux_cells(int, int) {}
-void initialize_microphysics_cells(int, int) {}
-void initialize_sign_cells(int, int, int) {}
-void compute_sign_cells(int, int, int, int) {}
-void compute_fluxes(int, int, int) {}
-void compute_west(int, int, int) {}
-void compute_west_uv(int, int, int) {}
-void compute_fluxes_moisture(int, int, int) {}
-void compute_west_moisture(int, int, int) {}
+// ---- fluxes.h: sign cells are evaluated inside the stage kernels --------------------------------------
+void compute_sign_cells(int, int, int, int) {}  // fluxes.cpp:316
 
 // ---- microphysics.h --------------------------------------------------------------------------------
-void init_microphysics(int, int) {}
-
 // run_microphysics, microphysics.cpp:97-132
 void run_microphysics(int il, int ih, int jl, int jh) {
     switch (MICROPHYSICS_OPTION) {
@@ -194,17 +186,17 @@ void zero_moisture(int il, int ih, int jl, int jh, int size) {  // microphysics.
 
 void microphysics_advance_inner(size_t) {  // microphysics.cpp:642-652
     if (!ok()) return;
-    switch_array(&qvs, &qvps);
-    switch_array(&qcs, &qcps);
-    switch_array(&qrs, &qrps);
+    swap_host(&qvs, &qvps);
+    swap_host(&qcs, &qcps);
+    swap_host(&qrs, &qrps);
     check(dyn_micro_advance_inner(g_ctx), "dyn_micro_advance_inner");
 }
 
 void microphysics_advance(size_t) {  // microphysics.cpp:657-669
     if (!ok()) return;
-    switch_array(&qvs, &qvps);
-    switch_array(&qcs, &qcps);
-    switch_array(&qrs, &qrps);
+    swap_host(&qvs, &qvps);
+    swap_host(&qcs, &qcps);
+    swap_host(&qrs, &qrps);
     check(dyn_micro_advance_outer(g_ctx), "dyn_micro_advance_outer");
 }
 
@@ -220,35 +212,25 @@ void integrate_non_hydro(double step, int il, int ih, int jl, int jh) {
 
 void integrate_hydro(double, int, int, int, int, int) { unsupported("integrate_hydro"); }
 
-void switch_array(double **var1, double **var2) { swap_host(var1, var2); }  // solver.cpp:229-235
-
 void advance_inner(size_t) {  // solver.cpp:241-248
     if (!ok()) return;
-    switch_array(&us, &ups);
-    switch_array(&vs, &vps);
-    switch_array(&ths, &thps);
-    switch_array(&ws, &wps);
+    swap_host(&us, &ups);
+    swap_host(&vs, &vps);
+    swap_host(&ths, &thps);
+    swap_host(&ws, &wps);
     check(dyn_advance_inner(g_ctx), "dyn_advance_inner");
 }
 
 void advance_outer(size_t) {  // solver.cpp:254-263: memcpy M<-P on the device, then the swap
     if (!ok()) return;
-    switch_array(&us, &ups);
-    switch_array(&vs, &vps);
-    switch_array(&ths, &thps);
-    switch_array(&ws, &wps);
+    swap_host(&us, &ups);
+    swap_host(&vs, &vps);
+    swap_host(&ths, &thps);
+    swap_host(&ws, &wps);
     check(dyn_advance_outer(g_ctx), "dyn_advance_outer");
 }
 
 // ---- boundaries.h / damping.h ------------------------------------------------------------------------
-void init_boundaries(int eastLength, int westLength, int northLength, int southLength, int halo) {  // boundaries.cpp:44
-    (void)eastLength;
-    (void)westLength;
-    if (halo > 0) return unsupported("init_boundaries (parallel halo)");
-    if (northLength != jnbuffer || southLength != jsbuffer)
-        fatal("init_boundaries: sponge widths differ from jnbuffer/jsbuffer used by initialize_pressure_solver");
-}
-
 void apply_boundary_condition(int) { CALL(dyn_apply_bc(g_ctx)); }
 void apply_boundary_condition_microphysics(int) { CALL(dyn_apply_bc_micro(g_ctx)); }
 void rayleigh_damping(int il, int ih, int jl, int jh, int level) { CALL(dyn_rayleigh(g_ctx, il, ih, jl, jh, level)); }

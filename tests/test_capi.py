@@ -47,11 +47,15 @@ def test_refapi_exports_the_reference_entry_points():
     header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
     cxx = header.split('extern "C"')[0]
     decls = re.findall(r"^\s*void\s+([a-z_0-9]+)\s*\(", cxx, flags=re.M)
-    assert "solve_pressure" in decls and "advect_uvw_velocity" in decls and "s_integrate_rk3" in decls
+    assert "solve_pressure" in decls and "advect_uvw_velocity" in decls and "rayleigh_damping" in decls
     for d in set(decls):
         assert re.search(rf" T {d}\(", out), d
     for d in _declared_c_functions("dyncore_refapi.h"):
         assert re.search(rf" T {d}$", out, flags=re.M), d
+    # the library defines no host globals and no time loop: the reference's own objects provide them
+    for g in ("us", "NX", "MICROPHYSICS_OPTION", "jnbuffer", "raydampheight"):
+        assert not re.search(rf" [BDC] {g}$", out, flags=re.M), g
+    assert "s_integrate_rk3" not in out
 
 
 def test_oracle_is_not_reachable_from_the_product():

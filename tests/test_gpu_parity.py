@@ -6,7 +6,6 @@ different FFT factorisation (rounding ~1e-15); Kessler uses the device exp/pow (
 of the FP64 plain form are held to 1e-11 relative (max-abs difference over the field's max-abs value)
 in every prognostic field.
 """
-import ctypes
 import os
 
 import numpy as np
@@ -14,7 +13,7 @@ import pytest
 
 from helpers import rel_err
 from model_amd import configs, ids
-from model_amd.core import DYN_FP32, DynCore, DynError, REFAPI_PATH
+from model_amd.core import DYN_FP32, DynCore, DynError
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -112,40 +111,57 @@ def test_supercell_reduced_grid_30_steps_fp64():
         assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
 
 
-def test_refapi_drop_in_driver_matches_oracle():
-    """The reference's own entry points (s_integrate_rk3 over solve_pressure, advect_uvw_velocity, ...)
-    from libdyncore_refapi drive the HIP library to the oracle's result."""
-    L = ctypes.CDLL(REFAPI_PATH)
-    L.dyn_refapi_setup.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_int] * 4
-    L.dyn_refapi_field.restype = ctypes.POINTER(ctypes.c_double)
-    L.dyn_refapi_column.restype = ctypes.POINTER(ctypes.c_double)
-    L.dyn_refapi_error.restype = ctypes.c_char_p
-    c = configs.supercell(nx=24, ny=24, nz=30)
-    L.dyn_refapi_exit_on_error(0)
-    L.dyn_refapi_setup(c.nx, c.ny, c.nz, c.dx, c.dz, c.dt, c.microphysics, c.rayleigh_level, c.sponge, c.sponge)
-    n3 = c.nx * c.ny * c.nz
+def _write_refshape_input(c, path):
+    with open(path, "wb") as fh:
+        np.array([c.nx, c.ny, c.nz, c.microphysics, c.rayleigh_level, c.sponge, c.sponge], dtype=np.int32).tofile(fh)
+        np.array([c.dx, c.dz, c.dt], dtype=np.float64).tofile(fh)
+        for name in ids.COLUMN_NAMES:
+            np.ascontiguousarray(c.columns[name], dtype=np.float64).tofile(fh)
+        for f, name in enumerate(ids.FIELD_NAMES):
+            arr = c.fields.get(name)
+            if arr is None:
+                arr = np.zeros((c.nx, c.ny) if f == ids.ACCRAIN else (c.nx, c.ny, c.nz))
+            for _ in range(3 if f in ids.PROGNOSTIC else 1):
+                np.ascontiguousarray(arr, dtype=np.float64).tofile(fh)
 
-    def host(f, role):
-        n = c.nx * c.ny if f == ids.ACCRAIN else n3
-        return np.ctypeslib.as_array(L.dyn_refapi_field(f, role), shape=(n,))
 
-    for name, arr in c.columns.items():
-        col = ids.COLUMN_ID[name]
-        np.ctypeslib.as_array(L.dyn_refapi_column(col), shape=(arr.size,))[:] = arr
-    for name, arr in c.fields.items():
-        f = ids.FIELD_ID[name]
-        for role in ((0, 1, 2) if f in ids.PROGNOSTIC else (0,)):
-            host(f, role)[:] = arr.ravel()
-    assert L.dyn_refapi_initialize() == 0, L.dyn_refapi_error()
-    assert L.dyn_refapi_run(12) == 0, L.dyn_refapi_error()
-    assert L.dyn_refapi_download() == 0
+def _read_refshape_output(c, path):
+    data = np.fromfile(path, dtype=np.float64)
+    n3, n2, pos, out = c.nx * c.ny * c.nz, c.nx * c.ny, 0, {}
+    for f in range(len(ids.FIELD_NAMES)):
+        for r in range(3 if f in ids.PROGNOSTIC else 1):
+            n = n2 if f == ids.ACCRAIN else n3
+            out[(f, r)] = data[pos:pos + n]
+            pos += n
+    assert pos == data.size
+    return out
+
+
+@pytest.mark.parametrize("name,steps", [("supercell", 12), ("warm_bubble", 20)])
+def test_reference_shaped_executable_matches_oracle(tmp_path, name, steps):
+    """An executable shaped like the reference's serial build (tests/refshape: every reference object
+    kept, the replaced entry points localized, the reference's own s_integrate_rk3 calling them by
+    name) drives the HIP library through libdyncore_refapi to the oracle's result, in every role."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "refshape", "_build", "refshape_driver")
+    c = configs.supercell(nx=24, ny=24, nz=30) if name == "supercell" else configs.warm_bubble(nx=32, ny=32, nz=20)
+    _write_refshape_input(c, tmp_path / "in.bin")
+    print(f"refshape_driver {name} {c.nx}x{c.ny}x{c.nz} {steps} steps", flush=True)
+    r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), str(steps)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _read_refshape_output(c, tmp_path / "out.bin")
     o = Oracle(c)
-    o.step(12)
-    for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR):
-        got = host(f, 1).reshape(c.nx, c.ny, c.nz)
-        e = rel_err(got, o.get_field(f, CUR))
-        assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
-    L.dyn_refapi_reset()
+    o.step(steps)
+    fields = [ids.U, ids.V, ids.W, ids.TH] + ([ids.QV, ids.QC, ids.QR] if c.microphysics else [])
+    for f in fields:
+        for role in (ids.ROLE_M, ids.ROLE_CUR, ids.ROLE_P):
+            ref = o.get_field(f, role)
+            e = rel_err(got[(f, role)].reshape(ref.shape), ref)
+            assert e <= 1e-11, (ids.FIELD_NAMES[f], role, e)
+    assert rel_err(got[(ids.PI, 0)].reshape(c.nx, c.ny, c.nz), o.get_field(ids.PI, 0)) <= 1e-11
+    if c.microphysics:
+        assert rel_err(got[(ids.ACCRAIN, 0)], o.get_field(ids.ACCRAIN, 0).ravel()) <= 1e-11
 
 
 def test_fp32_form_error_against_fp64_oracle():
