@@ -30,13 +30,24 @@ def build():
 
 
 def lib(variant: str = "parity"):
-    name = {"parity": "liboracle.so", "omp": "liboracle_omp.so"}[variant]
+    name = {"parity": "liboracle.so", "omp": "liboracle_omp.so", "tlm": "liboracle_tlm.so"}[variant]
     if variant in _LIBS:
         return _LIBS[variant]
     path = os.path.join(_HERE, "_build", name)
     if not os.path.exists(path):
         raise RuntimeError(f"oracle library {path} is not built (run `make -C oracle`)")
     L = ctypes.CDLL(path)
+    if variant == "tlm":
+        L.orc_tlm_create.restype = ctypes.c_void_p
+        L.orc_tlm_create.argtypes = [ctypes.POINTER(OrcGrid)]
+        L.orc_tlm_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_tlm_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_tlm_error.argtypes = [ctypes.c_void_p]
+        L.orc_tlm_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_tlm_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_tlm_set_column.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        _LIBS[variant] = L
+        return L
     L.orc_create.restype = ctypes.c_void_p
     L.orc_create.argtypes = [ctypes.POINTER(OrcGrid)]
     L.orc_destroy.argtypes = [ctypes.c_void_p]
@@ -136,3 +147,46 @@ class Oracle:
         fn = getattr(self.L, "orc_" + name)
         h = self.h
         return lambda *a: fn(h, *a)
+
+
+class Tangent:
+    """The oracle's tangent-linear model (liboracle_tlm.so): every field carries a tangent, and a step
+    advances both, differentiating the oracle's own arithmetic with its branches frozen at the base
+    state.  Fields are set with an optional tangent; get_field returns (value, tangent)."""
+
+    def __init__(self, case):
+        self.L = lib("tlm")
+        self.case = case
+        g = OrcGrid(case.nx, case.ny, case.nz, case.dx, case.dz, case.dt, case.microphysics,
+                    case.rayleigh_level, case.sponge, case.sponge)
+        self.h = self.L.orc_tlm_create(ctypes.byref(g))
+        from model_amd import ids
+        self.ids = ids
+        for name, arr in case.columns.items():
+            v = np.ascontiguousarray(arr, dtype=np.float64)
+            self.L.orc_tlm_set_column(self.h, ids.COLUMN_ID[name], _ptr(v))
+        for name, arr in case.fields.items():
+            f = ids.FIELD_ID[name]
+            for role in ((ids.ROLE_M, ids.ROLE_CUR, ids.ROLE_P) if f in ids.PROGNOSTIC else (0,)):
+                self.set_field(f, role, arr)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_tlm_destroy(self.h)
+            self.h = None
+
+    def set_field(self, f, role, value, tangent=None):
+        v = np.ascontiguousarray(value, dtype=np.float64)
+        t = None if tangent is None else np.ascontiguousarray(tangent, dtype=np.float64)
+        rc = self.L.orc_tlm_set_field(self.h, f, role, _ptr(v), None if t is None else _ptr(t))
+        assert rc == 0
+
+    def get_field(self, f, role=1):
+        c = self.case
+        shape = (c.nx, c.ny) if f == self.ids.ACCRAIN else (c.nx, c.ny, c.nz)
+        v, t = np.empty(shape), np.empty(shape)
+        assert self.L.orc_tlm_get_field(self.h, f, role, _ptr(v), _ptr(t)) == 0
+        return v, t
+
+    def step(self, n=1):
+        self.L.orc_tlm_step(self.h, n)
