@@ -198,7 +198,7 @@ def test_adjoint_with_half_slab_j_pass():
 def test_c4_full_size_gradient():
     """The adjoint configuration at full size: 192x192x40, dx = 20 km, dt = 60 s, 360 steps (6 h), a
     checkpoint every 20 steps, FP64; dJ/d(theta', u', v') at t = 0 along 16 seeded directions against
-    central differences (eps = 1e-6), and the Taylor remainder of one direction over eps = 1e-1 .. 1e-4.
+    central differences (eps = 1e-6), and the Taylor remainder of one direction over eps = 1e-2 .. 1e-4.
     Over 360 steps a few upwind signs flip inside +-eps somewhere in the trajectory, and each flip adds a
     first-order kink to the difference quotient, so the per-direction agreement is reported and bounded
     in distribution; the remainder must still fall as eps^2 (flips inside +-eps are themselves
@@ -234,10 +234,10 @@ def test_c4_full_size_gradient():
     print(f"C4: median rel {np.median(rels):.2e}, worst {rels.max():.2e}, within 1e-6: {(rels <= 1e-6).sum()}/16")
     J0 = J(first, 0.0)
     adj = float(sum(np.sum(g * d) for g, d in zip(grads, first)))
-    epss = [1e-1, 1e-2, 1e-3, 1e-4]
+    epss = [1e-2, 1e-3, 1e-4]
     rem = [abs(J(first, e) - J0 - e * adj) for e in epss]
     slopes = [np.log10(rem[i] / rem[i + 1]) for i in range(len(rem) - 1)]
     print("C4 Taylor remainders", ["%.3e" % r for r in rem], "slopes", ["%.2f" % s for s in slopes])
     core.close()
     assert np.median(rels) <= 1e-5 and rels.max() <= 1e-3
-    assert slopes[0] >= 1.7 and slopes[1] >= 1.7
+    assert all(1.8 <= s <= 2.4 for s in slopes)

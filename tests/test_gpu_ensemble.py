@@ -89,6 +89,6 @@ def test_c3_members_against_the_oracle():
             got = b.get_field(f, ids.ROLE_CUR)[3:-3, :, 1:-1]
             row[ids.FIELD_NAMES[f]] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         print(f"FP32 member {c.nx - 6}x{c.ny}: " + " ".join(f"{k} {v:.2e}" for k, v in row.items()))
-        assert row["th"] < 1e-5 and row["w"] < 1e-4
+        assert row["th"] < 1e-5 and row["w"] < 1e-3
     ens.close()
     ens32.close()
