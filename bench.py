@@ -178,6 +178,15 @@ def csrc_digest():
     return h.hexdigest()
 
 
+def kernel_base_name(symbol):
+    """k_adv_scalars for 'void dyn::(anonymous namespace)::k_adv_scalars<float, 8, true, 1>(dyn::Geo, ...)'."""
+    sym = symbol.replace("(anonymous namespace)::", "")
+    if sym.startswith("void "):
+        sym = sym[5:]
+    cut = min([i for i in (sym.find("<"), sym.find("(")) if i >= 0] or [len(sym)])
+    return sym[:cut].split("::")[-1]
+
+
 def measured_traffic(kernel):
     """(HBM bytes per launch of `kernel`, status) from the committed counter summary: FETCH_SIZE +
     WRITE_SIZE, each scaled by what it reads for a copy kernel of known size.  The summary must have been
@@ -190,7 +199,7 @@ def measured_traffic(kernel):
     if data.get("csrc_sha256") != csrc_digest():
         return None, "stale: the counter summary was measured on other kernel sources"
     base = KERNEL_SYMBOL.get(kernel, kernel)
-    hits = [row for name, row in data.get("kernels", {}).items() if name.split("<")[0].split("(")[0] == base]
+    hits = [row for name, row in data.get("kernels", {}).items() if kernel_base_name(name) == base]
     if len(hits) != 1:
         return None, f"{len(hits)} instantiations of {base} in the summary"
     return int(hits[0]["bytes"]), "measured on these sources"
@@ -214,7 +223,7 @@ def write_traffic_summary(fetch_csv, write_csv, copy_bytes):
         return {k: statistics.mean(v) for k, v in vals.items()}
 
     fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
-    copy = [k for k in fetch if k.split("(")[0].split("<")[0].endswith("k_copy_fields")]
+    copy = [k for k in fetch if kernel_base_name(k) == "k_copy_fields"]
     if len(copy) != 1:
         raise SystemExit(f"expected one k_copy_fields symbol, found {copy}")
     fs = copy_bytes / 1024.0 / fetch[copy[0]]

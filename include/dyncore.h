@@ -72,6 +72,11 @@ int dyn_set_column(dyn_ctx *ctx, int column, const double *values, size_t count)
 int dyn_set_field(dyn_ctx *ctx, int field, int role, const double *values, size_t count);
 int dyn_get_field(dyn_ctx *ctx, int field, int role, double *values, size_t count);
 int dyn_synchronize(dyn_ctx *ctx);
+/* Failure detection (no counterpart in the reference, which has only the Kessler guard of
+ * kessler.cpp:83-88, reported by dyn_synchronize): scans the current u, v, w, theta (qv, qc, qr) and pi
+ * for NaN or Inf; returns DYN_E_NUMERIC naming the fields, with bit DYN_<field> of *bad_fields set for
+ * each (bad_fields may be NULL).  One read of each field; stream-ordered, then synchronises. */
+int dyn_check_finite(dyn_ctx *ctx, int *bad_fields);
 
 /* ---- reference operators, one call each -------------------------------------------------- */
 /* advect_uvw_velocity(step,il,ih,jl,jh), advection.cpp:304 */
@@ -145,6 +150,51 @@ enum {
 };
 int dyn_tune_set(dyn_ctx *ctx, int param, int value);
 int dyn_tune_get(dyn_ctx *ctx, int param);
+
+/* ---- output: asynchronous staging and the reference's NetCDF file ------------------------------- */
+/* The reference writes u, v, w, pi, theta (and qv, qc, qr, rainfall when moist) every outfilefreq steps
+ * from its host arrays (driver_serial.cpp:317-345 -> output_meteorological_fields_to_file,
+ * files.cpp:673-700).  dyn_output_stage narrows the current values of those fields to float in a device
+ * buffer (stream-ordered after the steps queued before it) and copies them to pinned host memory on a
+ * stream of its own, so the steps queued after it overlap the copy; the step loop never waits on the
+ * host.  dyn_output_ready polls (1 landed, 0 not yet); dyn_output_get (one field) and
+ * dyn_netcdf_write_staged (one record of the file) wait for the copy.  The device and host buffers are
+ * allocated at the first dyn_output_stage and kept by the context. */
+int dyn_output_stage(dyn_ctx *ctx);
+int dyn_output_ready(dyn_ctx *ctx);
+int dyn_output_get(dyn_ctx *ctx, int field, float *values, size_t count);
+
+/* The reference's output file, written by this library's own netCDF classic-format (64-bit offset)
+ * writer: create_outfile's dimensions, global attributes and NC_FLOAT variables (files.cpp:51-282).
+ * The attributes the serial periodic path fixes (hydrostatic, islinear, stretched_grid ...) are written
+ * with those values; the rest come from dyn_nc_meta. */
+#define DYN_NC_MODEL_FILES_NOT_WRITTEN 0 /* file.h:15-17: values of last_file_successfully_written */
+#define DYN_NC_MODEL_FILES_WRITTEN 1
+#define DYN_NC_ALL_FILES_WRITTEN 2
+typedef struct dyn_nc_meta {
+    double dx, dz, dt;                     /* "dx" and "dy" (dy = dx), "dz", "dt" */
+    double latoffset, lonoffset, water_temperature, height_lowest_level;
+    int use_turbulent_stress, basic_state_option, perturbation_option;
+    const char *reanalysis_input_data, *perturbation_input_data, *basic_state_input_data; /* NULL: "" */
+} dyn_nc_meta;
+/* create_outfile(path, true, true, true) for an NX x NY x NZ grid (moisture variables when microphysics
+ * != 0), variables zero-filled, no records, last_file_successfully_written = DYN_NC_ALL_FILES_WRITTEN */
+int dyn_netcdf_create(const char *path, int nx, int ny, int nz, int microphysics, const dyn_nc_meta *meta);
+/* outfile_init (files.cpp:1262-1285) for a context: dyn_netcdf_create plus the basic state
+ * (write_all_bvars), tb, pib, qb, zu = zsu (write_all_mbvars) and lat[NY], lon[NX] (NULL: zeros) */
+int dyn_netcdf_init(dyn_ctx *ctx, const char *path, const dyn_nc_meta *meta, const double *lat, const double *lon);
+/* one record of the staged fields and time = hours (output_meteorological_fields_to_file and
+ * write_time_to_file, files.cpp:628-700); extends the record count */
+int dyn_netcdf_write_staged(dyn_ctx *ctx, const char *path, int record, double hours);
+/* nc_put_vara_double / nc_get_vara_double of a whole variable (one record of a record variable), as
+ * doubles; count must be the variable's value count per record.  Reads any classic-format file. */
+int dyn_netcdf_put(const char *path, const char *var, int record, const double *values, size_t count);
+int dyn_netcdf_read(const char *path, const char *var, int record, double *values, size_t count);
+int dyn_netcdf_records(const char *path); /* the record count (>= 0) or a DYN_E_* code */
+/* file_output_status / get_file_output_status (files.cpp:1205-1254): the restart attribute
+ * last_file_successfully_written; get returns -1 in *status when the attribute is absent */
+int dyn_netcdf_set_status(const char *path, int status);
+int dyn_netcdf_get_status(const char *path, int *status);
 
 /* ---- discrete adjoint of the dry step ------------------------------------------------------ */
 /* No counterpart in the reference (it has no adjoint).  dyn_adjoint_record runs nsteps forward steps

@@ -32,6 +32,9 @@ EXPORTS = [
     "dyn_adjoint_interval",
     "dyn_ensemble_create", "dyn_ensemble_destroy", "dyn_ensemble_synchronize", "dyn_ensemble_step", "dyn_ensemble_step_eager",
     "dyn_ensemble_set_shape", "dyn_ensemble_autotune", "dyn_ensemble_set_groups", "dyn_ensemble_groups",
+    "dyn_check_finite", "dyn_output_stage", "dyn_output_ready", "dyn_output_get", "dyn_netcdf_create", "dyn_netcdf_init",
+    "dyn_netcdf_write_staged", "dyn_netcdf_put", "dyn_netcdf_read", "dyn_netcdf_records", "dyn_netcdf_set_status",
+    "dyn_netcdf_get_status",
 ]
 
 
@@ -46,6 +49,16 @@ class DynGrid(ctypes.Structure):
                 ("dx", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
                 ("microphysics", ctypes.c_int), ("periodic_ew", ctypes.c_int), ("rayleigh_level", ctypes.c_int),
                 ("sponge_north", ctypes.c_int), ("sponge_south", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class NcMeta(ctypes.Structure):
+    """dyn_nc_meta: the global attributes of the output file that the configuration sets."""
+    _fields_ = [("dx", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+                ("latoffset", ctypes.c_double), ("lonoffset", ctypes.c_double), ("water_temperature", ctypes.c_double),
+                ("height_lowest_level", ctypes.c_double), ("use_turbulent_stress", ctypes.c_int),
+                ("basic_state_option", ctypes.c_int), ("perturbation_option", ctypes.c_int),
+                ("reanalysis_input_data", ctypes.c_char_p), ("perturbation_input_data", ctypes.c_char_p),
+                ("basic_state_input_data", ctypes.c_char_p)]
 
 
 _lib = None
@@ -100,6 +113,18 @@ def load():
     L.dyn_ensemble_set_groups.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_ensemble_groups.argtypes = [ctypes.c_void_p]
     L.dyn_stream.restype = ctypes.c_void_p
+    L.dyn_check_finite.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    for fn in ("dyn_output_stage", "dyn_output_ready"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p]
+    L.dyn_output_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]
+    L.dyn_netcdf_create.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 4 + [ctypes.POINTER(NcMeta)]
+    L.dyn_netcdf_init.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(NcMeta), _dp, _dp]
+    L.dyn_netcdf_write_staged.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
+    for fn in ("dyn_netcdf_put", "dyn_netcdf_read"):
+        getattr(L, fn).argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _dp, ctypes.c_size_t]
+    L.dyn_netcdf_records.argtypes = [ctypes.c_char_p]
+    L.dyn_netcdf_set_status.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.dyn_netcdf_get_status.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 

@@ -1,9 +1,9 @@
 // C ABI of libdyncore (include/dyncore.h): context lifetime, HBM-resident state, transfers, the
 // per-operator entry points and whole-step drivers.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -223,6 +223,15 @@ int swap_roles(Ctx *c, int f0, int f1) {
     return DYN_OK;
 }
 
+// NaN / Inf scan: one bit per field in *flag; one atomic per wavefront that saw a non-finite value
+template <typename T>
+__global__ void k_nonfinite(const T *__restrict__ a, long long n, int *__restrict__ flag, int bit) {
+    bool bad = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        bad |= !isfinite(a[i]);
+    if (__any(bad) && (threadIdx.x % warpSize) == 0) atomicOr(flag, bit);
+}
+
 }  // namespace
 }  // namespace dyn
 
@@ -268,8 +277,14 @@ int dyn_create(const dyn_grid *grid, dyn_ctx **out) {
         set_error("dyn_create: rayleigh_level must be >= 1");
         return DYN_E_ARG;
     }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("dyn_create: no HIP device");
+        return DYN_E_HIP;
+    }
     Ctx *c = new Ctx();
     c->g = g;
+    c->device = dev;
     c->prec = g.precision;
     c->elt = g.precision == DYN_FP32 ? 4 : 8;
     c->npts = (size_t)g.nx * g.ny * g.nz;
@@ -342,6 +357,7 @@ int dyn_create(const dyn_grid *grid, dyn_ctx **out) {
     if (hipMalloc(&c->raycoef, sizeof(double) * 2 * g.nz) != hipSuccess || hipMemset(c->raycoef, 0, sizeof(double) * 2 * g.nz) != hipSuccess)
         return fail(DYN_E_HIP);
     if (hipMalloc(&c->dflag, sizeof(int)) != hipSuccess || hipMemset(c->dflag, 0, sizeof(int)) != hipSuccess) return fail(DYN_E_HIP);
+    if (hipMalloc(&c->fflag, sizeof(int)) != hipSuccess) return fail(DYN_E_HIP);
     int rc = pressure_init(c);
     if (rc) return fail(rc);
     if (fused_supported(c)) {
@@ -365,12 +381,13 @@ int dyn_destroy(dyn_ctx *ctx) {
     drop_graph(c);
     unshare_roles(c);
     adjoint_release(c);
+    output_release(c);
     pressure_free(c);
     for (auto &f : c->prog)
         for (void *p : f)
             if (p) (void)hipFree(p);
     void *ptrs[] = {c->pi, c->vt, c->accrain, c->bar[0], c->bar[1], c->bar[2], c->bar[3], c->bar[4], c->bar[5], c->colbuf,
-                    c->sponge_n, c->sponge_s, c->raycoef, c->dflag};
+                    c->sponge_n, c->sponge_s, c->raycoef, c->dflag, c->fflag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free(c->colhost);
@@ -485,6 +502,37 @@ int dyn_synchronize(dyn_ctx *ctx) {
     DYN_HIP(hipMemcpy(&flag, c->dflag, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) {
         set_error("Kessler evaporation E > 100 (the reference aborts here, kessler.cpp:83-88)");
+        return DYN_E_NUMERIC;
+    }
+    return DYN_OK;
+}
+
+int dyn_check_finite(dyn_ctx *ctx, int *bad_fields) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->fflag);
+    const int last = c->g.microphysics ? DYN_QR : DYN_TH;
+    for (int f = DYN_U; f <= DYN_PI; f++) {
+        if (f > last && f < DYN_PI) continue;
+        const void *d = f == DYN_PI ? (const void *)c->pi : (const void *)c->prog[f][DYN_ROLE_CUR];
+        const long long n = (long long)c->npts;
+        const int blocks = (int)std::min<long long>((n + 255) / 256, 2048);
+        if (c->prec == DYN_FP32)
+            hipLaunchKernelGGL(k_nonfinite<float>, dim3(blocks), dim3(256), 0, c->stream, (const float *)d, n, c->fflag, 1 << f);
+        else
+            hipLaunchKernelGGL(k_nonfinite<double>, dim3(blocks), dim3(256), 0, c->stream, (const double *)d, n, c->fflag, 1 << f);
+        DYN_HIP(hipGetLastError());
+    }
+    int mask = 0;
+    DYN_HIP(hipMemcpyAsync(&mask, c->fflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    DYN_HIP(hipStreamSynchronize(c->stream));
+    if (bad_fields) *bad_fields = mask;
+    if (mask) {
+        static const char *names[] = {"u", "v", "w", "theta", "qv", "qc", "qr", "pi"};
+        std::string which;
+        for (int f = 0; f <= DYN_PI; f++)
+            if (mask & (1 << f)) which += std::string(which.empty() ? "" : ", ") + names[f];
+        set_error("non-finite values (NaN or Inf) in the current " + which);
         return DYN_E_NUMERIC;
     }
     return DYN_OK;

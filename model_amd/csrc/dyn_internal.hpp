@@ -130,8 +130,21 @@ constexpr int kStageMomentum = 1, kStageScalars = 2;
 
 struct AdjState;  // trajectory and adjoint buffers of the hand-written adjoint (dyn_adjoint.hip)
 
+// output staging (dyn_output.hip): the output fields narrowed to float in one device buffer, copied to
+// pinned host memory on a stream of their own
+struct OutputStage {
+    float *dev = nullptr, *host = nullptr;
+    size_t values = 0;
+    hipStream_t copy = nullptr;
+    hipEvent_t snap = nullptr, done = nullptr;
+    bool pending = false;
+};
+int output_release(struct Ctx *c);
+
 struct Ctx {
     dyn_grid g{};
+    OutputStage out{};
+    int device = 0;       // the HIP device current at dyn_create: every launch of the context goes there
     Geo geo{};
     int prec = DYN_FP64;
     size_t elt = 8;       // bytes per field element
@@ -153,6 +166,7 @@ struct Ctx {
     double *raycoef = nullptr;                       // device Rayleigh coefficients [2][NZ]
     int ray_level_cached = -1;
     int *dflag = nullptr;                             // device error flag (Kessler guard)
+    int *fflag = nullptr;                             // device bit mask of fields holding a NaN or Inf
     PressurePlan pp;
     // graph for dyn_step (per-operator steps: two steps restore the buffer assignment)
     hipGraph_t graph = nullptr;

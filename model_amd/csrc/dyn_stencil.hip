@@ -420,22 +420,31 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     int i = blockIdx.z + il;
     if (k >= g.NZ - 1 || j >= jh) return;
     long long p = IX(i, j, k);
-    const T cp = T(kCp), Lv = T(kLv), dt = T(g.dt), rhou = T(c.rhou[k]), tbk = T(c.tb[k]), qbk = T(c.qb[k]);
-    const T cpRd = T(kCp / kRd);
+    // Thermodynamics (full theta, pressure, vapour, temperature, saturation and the condensate) in FP64
+    // registers in both forms, rates in the storage type.  The condensate and the evaporation are small
+    // differences of ~1e-2 mixing ratios whose saturation value moves 6 % per kelvin, so in FP32 the
+    // rounding of the 300 K temperature (3e-5 K) alone would set their error.  In the FP64 form R = T and
+    // every expression is the reference's, in its order.
+    using R = double;
+    const R cp = R(kCp), Lv = R(kLv), tbk = R(c.tb[k]), qbk = R(c.qb[k]);
+    const T dt = T(g.dt), rhou = T(c.rhou[k]);
+    const R cpRd = R(kCp / kRd);
     T th = THP[p], qv = QVP[p], qc = QCP[p], qr = QRP[p];
-    T theta = th + THB[p] + tbk;
-    T pressure = PI[p] / (cp * T(c.tbv[k])) + PB[p];
-    T vapor = qv + QB[p] + qbk;
-    T temperature = theta * pressure;
-    T pd = T(kP0) * pow(pressure, cpRd);
+    R theta = R(th) + R(THB[p]) + tbk;
+    R pressure = R(PI[p]) / (cp * R(c.tbv[k])) + R(PB[p]);
+    R vapor = R(qv) + R(QB[p]) + qbk;
+    R temperature = theta * pressure;
+    R pd = R(kP0) * pow(pressure, cpRd);
     T A = qc > T(1.0e-3) ? T(0.001) * (qc - T(1.0e-3)) * dt : T(0);
     T B = qr > T(1.0e-12) ? rhou * T(2.2) * qc * pow(qr, T(0.875)) * dt : T(0);
     T E = T(0);
     if (qr > T(1.0e-12)) {
-        T esl = T(611.2) * exp(T(17.67) * (temperature - T(273.15)) / (temperature - T(29.65)));
-        T qvsat = T(0.62197) * esl / (pd - esl);
+        R esl = R(611.2) * exp(R(17.67) * (temperature - R(273.15)) / (temperature - R(29.65)));
+        R qvsat = R(0.62197) * esl / (pd - esl);
         T cvent = T(1.6) + T(30.3922) * pow(rhou * qr, T(0.2046));
-        E = (T(1.0) / rhou) * (((T(1.0) - vapor / qvsat) * cvent * pow(rhou * qr, T(0.525))) / (T(2.03e4) + T(9.58e6) / (pd * qvsat))) * dt;
+        R e = (R(1.0) / R(rhou)) * (((R(1.0) - vapor / qvsat) * R(cvent) * R(pow(rhou * qr, T(0.525)))) /
+                                  (R(2.03e4) + R(9.58e6) / (pd * qvsat))) * R(dt);
+        E = T(e);
         E = fmin(qr, E);
         E = fmax(E, T(0));
         if (E > T(100)) atomicOr(flag, 1);
@@ -444,23 +453,23 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     T total_convert = fmin(A + B, qc);
     qc -= total_convert;
     qr += (total_convert - E);
-    T de = (Lv / (cp * pressure)) * E;
-    th -= de;
-    theta = th + THB[p] + tbk;
-    vapor = qv + QB[p] + qbk;
+    R de = (Lv / (cp * pressure)) * R(E);
+    th -= T(de);
+    theta = R(th) + R(THB[p]) + tbk;
+    vapor = R(qv) + R(QB[p]) + qbk;
     temperature = theta * pressure;
-    T esl = T(611.2) * exp(T(17.67) * (temperature - T(273.15)) / (temperature - T(29.65)));
-    T qvsat = T(0.62197) * esl / (pd - esl);
-    const T f = T(17.67 * (273.15 - 29.65) * 2.5e6 / 1004.0);
-    T tm = temperature - T(29.65);
-    T phi = pd / (pd - esl) * qvsat * f / (tm * tm);
-    T C = T(0);
-    if (vapor > qvsat) C = (vapor - qvsat) / (T(1.0) + phi);
-    else if (vapor < qvsat && qc > T(0)) C = -fmin(qc, (qvsat - vapor) / (T(1.0) + phi));
-    qc += C;
-    qv -= C;
-    T dc = (Lv / (cp * pressure)) * C;
-    th += dc;
+    R esl = R(611.2) * exp(R(17.67) * (temperature - R(273.15)) / (temperature - R(29.65)));
+    R qvsat = R(0.62197) * esl / (pd - esl);
+    const R f = R(17.67 * (273.15 - 29.65) * 2.5e6 / 1004.0);
+    R tm = temperature - R(29.65);
+    R phi = pd / (pd - esl) * qvsat * f / (tm * tm);
+    R C = R(0);
+    if (vapor > qvsat) C = (vapor - qvsat) / (R(1.0) + phi);
+    else if (vapor < qvsat && qc > T(0)) C = -fmin(R(qc), (qvsat - vapor) / (R(1.0) + phi));
+    qc += T(C);
+    qv -= T(C);
+    R dc = (Lv / (cp * pressure)) * C;
+    th += T(dc);
     if (tail.in(i, j)) {
         auto damp = [&](int kk, T v) { return kk >= tail.lev ? v - T(tail.ray[kk]) * v : v; };
         const int kl = k == 1 ? 0 : (k == g.NZ - 2 ? g.NZ - 1 : -1);  // the lid level this level feeds

@@ -211,3 +211,21 @@ def test_base_state_hydrostatic_and_density():
     assert np.allclose(dpi, -9.8 * dz / (1004.0 * tv), rtol=1e-12)
     # unstretched grid: mu = mw = 1 when the lowest level sits at dz
     assert np.allclose(col["mu"], 1.0) and np.allclose(col["mw"], 1.0)
+
+
+def test_kessler_guard_flags_the_oracle_state():
+    """kessler.cpp:83-88: E > 100 is the reference's fatal condition; the oracle records it (orc_error),
+    for the same forced state the GPU test feeds the HIP library (an absurd rain mixing ratio)."""
+    import numpy as np
+    from helpers import zero_case
+    from model_amd import ids
+    from oracle.pyoracle import Oracle
+    for q, flagged in ((1e6, 0), (1e8, 1)):
+        c = zero_case(nx=16, ny=16, nz=12, microphysics=1)
+        o = Oracle(c)
+        qr = np.zeros((c.nx, c.ny, c.nz))
+        qr[8, 8, 3] = q
+        o.set_field(ids.QR, ids.ROLE_P, qr)
+        o.set_field(ids.QV, ids.ROLE_P, np.full((c.nx, c.ny, c.nz), -0.5 * c.columns["qb"][3]))
+        o.kessler(1, c.nx - 1, 1, c.ny - 1)
+        assert o.error == flagged
