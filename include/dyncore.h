@@ -77,6 +77,13 @@ int dyn_synchronize(dyn_ctx *ctx);
  * for NaN or Inf; returns DYN_E_NUMERIC naming the fields, with bit DYN_<field> of *bad_fields set for
  * each (bad_fields may be NULL).  One read of each field; stream-ordered, then synchronises. */
 int dyn_check_finite(dyn_ctx *ctx, int *bad_fields);
+/* dyn_get_field without any device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving or output loops, harnesses that forbid hipMalloc).  An
+ * FP32 field is widened to double in a file-scope device array of 8,388,608 doubles (64 MiB, one per
+ * GPU, held until the process ends); fields of more values return DYN_E_ARG (use dyn_get_field).  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (from several host threads)
+ * race.  FP64 fields copy straight to the host.  Same values as dyn_get_field, bit for bit. */
+int dyn_get_field_noalloc(dyn_ctx *ctx, int field, int role, double *values, size_t count);
 
 /* ---- reference operators, one call each -------------------------------------------------- */
 /* advect_uvw_velocity(step,il,ih,jl,jh), advection.cpp:304 */

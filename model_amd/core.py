@@ -32,7 +32,7 @@ EXPORTS = [
     "dyn_adjoint_interval",
     "dyn_ensemble_create", "dyn_ensemble_destroy", "dyn_ensemble_synchronize", "dyn_ensemble_step", "dyn_ensemble_step_eager",
     "dyn_ensemble_set_shape", "dyn_ensemble_autotune", "dyn_ensemble_set_groups", "dyn_ensemble_groups",
-    "dyn_check_finite", "dyn_output_stage", "dyn_output_ready", "dyn_output_get", "dyn_netcdf_create", "dyn_netcdf_init",
+    "dyn_get_field_noalloc", "dyn_check_finite", "dyn_output_stage", "dyn_output_ready", "dyn_output_get", "dyn_netcdf_create", "dyn_netcdf_init",
     "dyn_netcdf_write_staged", "dyn_netcdf_put", "dyn_netcdf_read", "dyn_netcdf_records", "dyn_netcdf_set_status",
     "dyn_netcdf_get_status",
 ]
@@ -113,6 +113,7 @@ def load():
     L.dyn_ensemble_set_groups.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dyn_ensemble_groups.argtypes = [ctypes.c_void_p]
     L.dyn_stream.restype = ctypes.c_void_p
+    L.dyn_get_field_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_size_t]
     L.dyn_check_finite.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     for fn in ("dyn_output_stage", "dyn_output_ready"):
         getattr(L, fn).argtypes = [ctypes.c_void_p]

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -13,6 +14,34 @@ namespace dyn {
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+// Device staging of dyn_get_field_noalloc: an FP32 field is widened to double here before its copy to
+// the host, so the call allocates nothing.  One array per GPU (a file-scope __device__ array), held until
+// the process ends; every call on a GPU shares it, so calls that overlap on one GPU race.  Nothing needs
+// clearing: each call overwrites the values it copies out.
+constexpr size_t kNoallocScratchDoubles = size_t(8) << 20;  // 64 MiB: a field of up to 8,388,608 values
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *noalloc_scratch_ptr[kMaxDevices] = {};
+static std::mutex noalloc_scratch_mutex;
+
+// once per device, with that device current (dyn_create calls it): the array's address on the device.
+// In host code `scratch` names a host-side stand-in, never usable as a device address.
+static int noalloc_scratch_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("dyn_create: device index beyond the staging table");
+        return DYN_E_ARG;
+    }
+    std::lock_guard<std::mutex> lock(noalloc_scratch_mutex);
+    if (noalloc_scratch_ptr[dev]) return DYN_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        set_error("dyn_create: hipGetSymbolAddress of the staging array failed");
+        return DYN_E_HIP;
+    }
+    noalloc_scratch_ptr[dev] = static_cast<double *>(p);
+    return DYN_OK;
+}
 
 static hipEvent_t take_event(Ctx *c) {
     if (!c->pool.empty()) {
@@ -282,6 +311,7 @@ int dyn_create(const dyn_grid *grid, dyn_ctx **out) {
         set_error("dyn_create: no HIP device");
         return DYN_E_HIP;
     }
+    if (int rc = noalloc_scratch_setup(dev)) return rc;
     Ctx *c = new Ctx();
     c->g = g;
     c->device = dev;
@@ -479,6 +509,41 @@ static int transfer(Ctx *c, int field, int role, double *host, size_t count, boo
     if (hipStreamSynchronize(c->stream) != hipSuccess) rc = DYN_E_HIP;
     (void)hipFree(tmp);
     if (rc) set_error("FP32 transfer failed");
+    return rc;
+}
+
+int dyn_get_field_noalloc(dyn_ctx *ctx, int field, int role, double *values, size_t count) {
+    Ctx *c = as_ctx(ctx);
+    CHECK_CTX(c);
+    if (field < 0 || field >= DYN_FIELD_COUNT || !values) {
+        set_error("dyn_get_field_noalloc: bad field id");
+        return DYN_E_ARG;
+    }
+    void *d = field_ptr(c, field, role);
+    if (!d) {
+        set_error("dyn_get_field_noalloc: field not present in this context (moisture fields need microphysics=1) or bad role");
+        return DYN_E_ARG;
+    }
+    size_t n = field_count(c, field);
+    if (count != n) {
+        set_error("dyn_get_field_noalloc: expected " + std::to_string(n) + " values, got " + std::to_string(count));
+        return DYN_E_ARG;
+    }
+    if (c->prec == DYN_FP64) {  // no staging: the field is already double
+        DYN_HIP(hipMemcpyAsync(values, d, n * 8, hipMemcpyDeviceToHost, c->stream));
+        DYN_HIP(hipStreamSynchronize(c->stream));
+        return DYN_OK;
+    }
+    if (n > kNoallocScratchDoubles) {
+        set_error("dyn_get_field_noalloc: " + std::to_string(n) + " values exceed the " +
+                  std::to_string(kNoallocScratchDoubles) + "-value device staging array; use dyn_get_field");
+        return DYN_E_ARG;
+    }
+    double *stage = noalloc_scratch_ptr[c->device];  // the array's device address on this GPU
+    int rc = launch_convert(c, stage, (const double *)d, n, false);
+    if (!rc && hipMemcpyAsync(values, stage, n * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = DYN_E_HIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = DYN_E_HIP;
+    if (rc) set_error("dyn_get_field_noalloc: FP32 read-back failed");
     return rc;
 }
 
