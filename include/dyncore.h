@@ -111,6 +111,29 @@ int dyn_advance_outer(dyn_ctx *ctx);
 int dyn_micro_advance_inner(dyn_ctx *ctx);
 int dyn_micro_advance_outer(dyn_ctx *ctx);
 
+/* ---- turbulence parameterization (USE_TURBULENT_STRESS, turbulence.cpp) ------------------------ */
+/* Switch the Smagorinsky-Lilly closure on or off (initialize_landsea + init_kmix, turbulence.cpp:85-147,
+ * 370-375, with the vertical mixing length from the zsu column).  landsea holds NX*NY values (< 0.5:
+ * land, fixed drag 3e-3; otherwise ocean, 1.1e-3 + 4e-5 |V|) or is NULL for all ocean.  With it on, a
+ * step computes the coefficients and tendencies at its start and adds them in every stage; steps take
+ * the per-operator sequence (the fused stage kernels carry no turbulence), and the adjoint and
+ * ensembles refuse the context.  Drops the captured step graphs. */
+int dyn_set_turbulence(dyn_ctx *ctx, int on, const double *landsea, size_t count);
+/* calculate_diff_tend(il,ih,jl,jh), turbulence.cpp:153-226 (the serial driver calls it over
+ * [1,NX-1) x [1,NY-1), driver_serial.cpp:194): mixing coefficients from the M buffers, then the
+ * friction and scalar diffusion tendencies */
+int dyn_calculate_diff_tend(dyn_ctx *ctx, int il, int ih, int jl, int jh);
+/* apply_velocity_diffusion(step,il,ih,jl,jh), turbulence.cpp:233-258 (solver.cpp:104) */
+int dyn_apply_velocity_diffusion(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh);
+/* apply_scalar_diffusion(step,il,ih,jl,jh), turbulence.cpp:263-273 (driver_serial.cpp:218) */
+int dyn_apply_scalar_diffusion(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh);
+/* apply_moisture_diffusion(step,il,ih,jl,jh), turbulence.cpp:278-292 (driver_serial.cpp:231) */
+int dyn_apply_moisture_diffusion(dyn_ctx *ctx, double step, int il, int ih, int jl, int jh);
+/* read one turbulence array as FP64 (NX*NY*NZ values): 0 Kmix, 1 KHmix, 2-4 u/v/w friction, 5-8
+ * theta/qv/qc/qr diffusion tendencies (turbulence.cpp:32-47), 9 Kessler's saturation flag (0 or 1,
+ * isSaturated, kessler.cpp:129-150) */
+int dyn_get_turbulence(dyn_ctx *ctx, int which, double *values, size_t count);
+
 /* ---- whole steps ------------------------------------------------------------------------- */
 /* s_integrate_rk3 (driver_serial.cpp:178-286) as the sequence of per-operator calls above */
 int dyn_step_reference(dyn_ctx *ctx, int nsteps);

@@ -5,11 +5,11 @@
  *
  * Linking: the reference keeps every one of its objects.  In the objects that define the functions
  * declared below (pressure.o, advection.o, fluxes.o, kessler.o, microphysics.o, boundaries.o,
- * damping.o, solver.o) these symbols are made local with objcopy --localize-symbols, using the list
+ * damping.o, solver.o, turbulence.o) these symbols are made local with objcopy --localize-symbols, using the list
  * the build writes next to the library (libdyncore_refapi.replaced: the C++ symbols this library
  * defines); every call from another object then binds to this library.  Everything else in those
  * objects stays as it is (init_boundaries, init_damping, init_microphysics, precip_rate, the fall-speed
- * functions, the output code ...).
+ * functions, init_kmix, initialize_landsea, the output code ...).
  *
  * State: the reference keeps every field in host globals (model.h:63-91, microphysics.h:18-25), which
  * its own initializer.cpp defines; this library only references them.  The fields live in HBM,
@@ -17,8 +17,8 @@
  * initialize_serial makes, driver_serial.cpp:58) and copied back by dyn_refapi_download().  Host
  * pointer swaps stay in lock-step with the device buffer rotation.
  *
- * Options outside the implemented path (hydrostatic, ice, linear, tracers, semi-Lagrangian rain,
- * turbulence) report through dyn_refapi_error() and abort the way the reference aborts on an
+ * Options outside the implemented path (hydrostatic, ice, linear, tracers, semi-Lagrangian rain)
+ * report through dyn_refapi_error() and abort the way the reference aborts on an
  * unsupported option: printf + exit(0).
  */
 #ifndef DYNCORE_REFAPI_H
@@ -62,6 +62,13 @@ void integrate_non_hydro(double step, int il, int ih, int jl, int jh);  /* solve
 void integrate_hydro(double step, int itr, int il, int ih, int jl, int jh); /* solver.cpp:35 (unsupported) */
 void advance_inner(size_t num_bytes);                                   /* solver.cpp:241 */
 void advance_outer(size_t num_bytes);                                   /* solver.cpp:254 */
+
+/* ---- turbulence.h:8-11 (USE_TURBULENT_STRESS; the closure's state is created by
+ * initialize_pressure_solver from the land-sea mask initialize_landsea left) ---------------------- */
+void calculate_diff_tend(int il, int ih, int jl, int jh);                  /* turbulence.cpp:158 */
+void apply_velocity_diffusion(double step, int il, int ih, int jl, int jh); /* turbulence.cpp:233 */
+void apply_scalar_diffusion(double step, int il, int ih, int jl, int jh);   /* turbulence.cpp:263 */
+void apply_moisture_diffusion(double step, int il, int ih, int jl, int jh); /* turbulence.cpp:278 */
 
 /* ---- boundaries.h / damping.h ------------------------------------------------------------------ */
 void apply_boundary_condition(int mpi_proc_null);                       /* boundaries.cpp:119 */

@@ -19,7 +19,7 @@ LIB = os.path.join(OUT, "libdyncore.so")
 REFAPI_LIB = os.path.join(OUT, "libdyncore_refapi.so")
 
 HIP_SOURCES = ["dyn_capi.hip", "dyn_stencil.hip", "dyn_pressure.hip", "dyn_fused.hip", "dyn_adjoint.hip", "dyn_ensemble.hip",
-               "dyn_output.hip"]
+               "dyn_output.hip", "dyn_turbulence.hip"]
 REFAPI_SOURCES = ["refapi/refapi.cpp"]
 # the C++ symbols libdyncore_refapi.so defines: what a maintainer localizes in the reference's objects
 REFAPI_REPLACED = os.path.join(OUT, "libdyncore_refapi.replaced")

@@ -34,7 +34,8 @@ EXPORTS = [
     "dyn_ensemble_set_shape", "dyn_ensemble_autotune", "dyn_ensemble_set_groups", "dyn_ensemble_groups",
     "dyn_get_field_noalloc", "dyn_check_finite", "dyn_output_stage", "dyn_output_ready", "dyn_output_get", "dyn_netcdf_create", "dyn_netcdf_init",
     "dyn_netcdf_write_staged", "dyn_netcdf_put", "dyn_netcdf_read", "dyn_netcdf_records", "dyn_netcdf_set_status",
-    "dyn_netcdf_get_status",
+    "dyn_netcdf_get_status", "dyn_set_turbulence", "dyn_calculate_diff_tend", "dyn_apply_velocity_diffusion",
+    "dyn_apply_scalar_diffusion", "dyn_apply_moisture_diffusion", "dyn_get_turbulence",
 ]
 
 
@@ -126,6 +127,11 @@ def load():
     L.dyn_netcdf_records.argtypes = [ctypes.c_char_p]
     L.dyn_netcdf_set_status.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.dyn_netcdf_get_status.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+    L.dyn_set_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_size_t]
+    L.dyn_calculate_diff_tend.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
+    for fn in ("dyn_apply_velocity_diffusion", "dyn_apply_scalar_diffusion", "dyn_apply_moisture_diffusion"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4
+    L.dyn_get_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_size_t]
     _lib = L
     return L
 
@@ -191,6 +197,21 @@ class DynCore:
 
     def sync(self):
         _check(self.L.dyn_synchronize(self.h))
+
+    def set_turbulence(self, on=True, landsea=None):
+        """USE_TURBULENT_STRESS on or off; landsea: NX x NY mask (< 0.5 land) or None for all ocean."""
+        if landsea is None:
+            _check(self.L.dyn_set_turbulence(self.h, int(on), None, 0))
+        else:
+            ls = np.ascontiguousarray(landsea, dtype=np.float64)
+            _check(self.L.dyn_set_turbulence(self.h, int(on), _ptr(ls), ls.size))
+
+    def get_turbulence(self, which):
+        """0 Kmix, 1 KHmix, 2-4 u/v/w friction, 5-8 theta/qv/qc/qr diffusion tendencies, 9 saturation flag."""
+        c = self.case
+        out = np.empty((c.nx, c.ny, c.nz))
+        _check(self.L.dyn_get_turbulence(self.h, which, _ptr(out), out.size))
+        return out
 
     def step(self, n=1):
         _check(self.L.dyn_step(self.h, n))

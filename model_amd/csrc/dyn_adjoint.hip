@@ -833,6 +833,10 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
         set_error("dyn_adjoint_record: the adjoint covers the dry step (microphysics option 0)");
         return DYN_E_UNSUPPORTED;
     }
+    if (c->turb.on) {
+        set_error("dyn_adjoint_record: the adjoint covers the step without turbulence");
+        return DYN_E_UNSUPPORTED;
+    }
     if (nsteps <= 0 || every < 0) {
         set_error("dyn_adjoint_record: nsteps must be positive and every non-negative");
         return DYN_E_ARG;

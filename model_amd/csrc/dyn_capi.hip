@@ -76,9 +76,8 @@ int launch_zero_moisture_shell(Ctx *c, int il, int ih, int jl, int jh);
 int launch_post_stage(Ctx *c, double step, bool pgf, bool scalars, bool clip, bool uvw, int ray_level = -1,
                       bool interior_done = false);
 
-namespace {
-
-// the captured step graph bakes in launch shapes: drop it when a knob changes
+// the captured step graph bakes in launch shapes and the kernel sequence: drop it when a knob or the
+// turbulence switch changes
 void drop_graph(Ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -91,6 +90,8 @@ void drop_graph(Ctx *c) {
     }
     c->graphs.clear();
 }
+
+namespace {
 
 
 
@@ -412,6 +413,7 @@ int dyn_destroy(dyn_ctx *ctx) {
     unshare_roles(c);
     adjoint_release(c);
     output_release(c);
+    turb_release(c);
     pressure_free(c);
     for (auto &f : c->prog)
         for (void *p : f)
@@ -759,7 +761,8 @@ int dyn_micro_advance_outer(dyn_ctx *ctx) {
     return swap_roles(c, DYN_QV, DYN_QR);
 }
 
-// s_integrate_rk3, driver_serial.cpp:178-286 (no diffusion, turbulence, terrain or heating)
+// s_integrate_rk3, driver_serial.cpp:178-286 (no diffusion, terrain or heating; turbulence when
+// dyn_set_turbulence turned it on)
 static int reference_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nullptr) {
     Ctx *c = as_ctx(ctx);
     const double steps[] = {1. / 3., 0.5, 1.0};
@@ -770,16 +773,21 @@ static int reference_step(dyn_ctx *ctx, StageHook hook = nullptr, void *arg = nu
     do {                       \
         if ((rc = (x))) return rc; \
     } while (0)
+    const bool turb = c->turb.on;
+    if (turb) RC(turb_step_begin(c));  // calculate_diff_tend(1,NX-1,1,NY-1), driver_serial.cpp:194
     for (int s = 0; s < 3; s++) {
         double step = steps[s];
         if (hook) RC(hook(c, s, arg));
         // integrate_non_hydro(step,0,NX,0,NY), solver.cpp:87-138
         RC(dyn_advect_uvw(ctx, step, 3, NX - 3, 3, NY - 3));
+        if (turb) RC(turb_apply_velocity(c, step));  // solver.cpp:104
         RC(dyn_solve_pressure(ctx, step));
         RC(dyn_pgf(ctx, step, 1, NX - 1, 1, NY - 1));
         RC(dyn_advect_theta(ctx, step, 3, NX - 3, 3, NY - 3));
+        if (turb) RC(turb_apply_scalars(c, step, true));  // driver_serial.cpp:218
         if (moist) {
             RC(dyn_advect_moisture(ctx, step, 3, NX - 3, 3, NY - 3));
+            if (turb) RC(turb_apply_scalars(c, step, false));  // driver_serial.cpp:231
             RC(dyn_zero_moisture(ctx, 1, NX - 1, 1, NY - 1));
         }
         RC(dyn_apply_bc(ctx));

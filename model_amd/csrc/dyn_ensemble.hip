@@ -283,6 +283,10 @@ int dyn_ensemble_create(dyn_ctx *const *members, int n, dyn_ensemble **out) {
             set_error("dyn_ensemble_create: ensemble members must be dry (microphysics 0)");
             return DYN_E_UNSUPPORTED;
         }
+        if (c->turb.on) {
+            set_error("dyn_ensemble_create: ensemble members step without turbulence");
+            return DYN_E_UNSUPPORTED;
+        }
         if (!fused_supported(c) || fused_waves(c) != 1) {
             set_error("dyn_ensemble_create: members need NZ <= 64");
             return DYN_E_UNSUPPORTED;

@@ -1146,7 +1146,10 @@ int fused_waves(const Ctx *c) {
 }
 
 // ... and 32-bit byte offsets into every field
-bool fused_supported(const Ctx *c) { return fused_waves(c) > 0 && (double)c->npts * (double)c->elt < 4294967296.0; }
+// the fused stage kernels carry no turbulence: with it on, steps take the per-operator sequence
+bool fused_supported(const Ctx *c) {
+    return !c->turb.on && fused_waves(c) > 0 && (double)c->npts * (double)c->elt < 4294967296.0;
+}
 
 namespace {
 template <typename T, int KW>

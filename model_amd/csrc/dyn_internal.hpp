@@ -141,9 +141,21 @@ struct OutputStage {
 };
 int output_release(struct Ctx *c);
 
+// turbulence parameterization (dyn_turbulence.hip): buf holds Kmix, KHmix, the u/v/w friction and the
+// theta/qv/qc/qr diffusion tendencies (storage type, zero outside the interior they are computed on);
+// sat is Kessler's saturation flag per point
+struct Turbulence {
+    bool on = false;
+    void *buf[9] = {nullptr};
+    unsigned char *sat = nullptr;
+    double *landsea = nullptr, *vml2 = nullptr, *vkmax = nullptr;
+    double kmixh_max = 0, lH = 0;
+};
+
 struct Ctx {
     dyn_grid g{};
     OutputStage out{};
+    Turbulence turb{};
     int device = 0;       // the HIP device current at dyn_create: every launch of the context goes there
     Geo geo{};
     int prec = DYN_FP64;
@@ -219,6 +231,12 @@ int launch_bc(Ctx *c, bool micro);
 int launch_periodic(Ctx *c, void *const *fields, int nfields);
 int launch_kessler(Ctx *c, int il, int ih, int jl, int jh);
 int launch_kessler_step_end(Ctx *c, int ray_level);
+// turbulence: tendencies at the start of a step, and their addition to the forecasts in a stage
+void turb_release(Ctx *c);
+int turb_step_begin(Ctx *c);
+int turb_apply_velocity(Ctx *c, double step);
+int turb_apply_scalars(Ctx *c, double step, bool theta);
+void drop_graph(Ctx *c);
 int launch_rayleigh(Ctx *c, int il, int ih, int jl, int jh, int level);
 int rayleigh_table(Ctx *c, int level);
 int pressure_init(Ctx *c);

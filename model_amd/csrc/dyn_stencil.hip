@@ -413,8 +413,8 @@ struct KessTail {
 template <typename T>
 __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ THP, T *__restrict__ QVP, T *__restrict__ QCP,
                                                  T *__restrict__ QRP, const T *__restrict__ THB, const T *__restrict__ QB,
-                                                 const T *__restrict__ PB, const T *__restrict__ PI, int *__restrict__ flag, int il,
-                                                 int jl, int jh, KessTail tail) {
+                                                 const T *__restrict__ PB, const T *__restrict__ PI, int *__restrict__ flag,
+                                                 unsigned char *__restrict__ sat, int il, int jl, int jh, KessTail tail) {
     int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y * blockDim.y + threadIdx.y + jl;
     int i = blockIdx.z + il;
@@ -464,8 +464,16 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     R tm = temperature - R(29.65);
     R phi = pd / (pd - esl) * qvsat * f / (tm * tm);
     R C = R(0);
-    if (vapor > qvsat) C = (vapor - qvsat) / (R(1.0) + phi);
-    else if (vapor < qvsat && qc > T(0)) C = -fmin(R(qc), (qvsat - vapor) / (R(1.0) + phi));
+    // sat: isSaturated of kessler.cpp:129-150, for the turbulence's moist stability (nullptr when off)
+    if (vapor > qvsat) {
+        C = (vapor - qvsat) / (R(1.0) + phi);
+        if (sat) sat[p] = 1;
+    } else if (vapor < qvsat && qc > T(0)) {
+        C = -fmin(R(qc), (qvsat - vapor) / (R(1.0) + phi));
+        if (sat) sat[p] = -C < R(qc) ? 1 : 0;
+    } else if (sat) {
+        sat[p] = vapor == qvsat ? 1 : 0;
+    }
     qc += T(C);
     qv -= T(C);
     R dc = (Lv / (cp * pressure)) * C;
@@ -669,8 +677,8 @@ int kessler_t(Ctx *c, int il, int ih, int jl, int jh, KessTail tail = KessTail{}
     {
         Timed _t(c, "kessler", (double)(ih - il) * (jh - jl) * (c->geo.NZ - 2));
         hipLaunchKernelGGL(k_kessler<T>, grd, blk, 0, c->stream, c->geo, c->cols, P(DYN_TH), P(DYN_QV), P(DYN_QC), P(DYN_QR),
-                       (const T *)c->bar[3], (const T *)c->bar[4], (const T *)c->bar[5], (const T *)c->pi, c->dflag, il, jl, jh,
-                           tail);
+                       (const T *)c->bar[3], (const T *)c->bar[4], (const T *)c->bar[5], (const T *)c->pi, c->dflag, c->turb.on ? c->turb.sat : nullptr, il,
+                           jl, jh, tail);
     }
     DYN_LAUNCH_CHECK();
     {

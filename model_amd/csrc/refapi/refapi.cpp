@@ -138,6 +138,9 @@ void initialize_pressure_solver() {
     double *cols[DYN_COLUMN_COUNT] = {zu, zw, zsu, zsw, mu, mw, rhou, rhow, one_d_rhou, one_d_rhow, tb, tbv, tbw, qb, pib, f};
     for (int k = 0; k < DYN_COLUMN_COUNT; k++) CALL(dyn_set_column(g_ctx, k, cols[k], k == DYN_COL_F ? (size_t)NY : (size_t)NZ));
     CALL(dyn_refapi_upload());
+    // the reference's initialize_serial runs init_kmix and initialize_landsea before this call
+    // (driver_serial.cpp:52-58): the closure's device state starts here, from the mask those left
+    if (USE_TURBULENT_STRESS) CALL(dyn_set_turbulence(g_ctx, 1, landsea, landsea ? (size_t)NX * NY : 0));
 }
 
 void solve_pressure(double step) { CALL(dyn_solve_pressure(g_ctx, step)); }  // pressure.cpp:142
@@ -205,7 +208,7 @@ void microphysics_advance(size_t) {  // microphysics.cpp:657-669
 void integrate_non_hydro(double step, int il, int ih, int jl, int jh) {
     const int buffer = 1;
     advect_uvw_velocity(step, il + 3, ih - 3, jl + 3, jh - 3);
-    if (USE_TURBULENT_STRESS) return unsupported("apply_velocity_diffusion");
+    if (USE_TURBULENT_STRESS) apply_velocity_diffusion(step, il + 3, ih - 3, jl + 3, jh - 3);  // solver.cpp:103
     solve_pressure(step);
     CALL(dyn_pgf(g_ctx, step, il + buffer, ih - buffer, jl + buffer, jh - buffer));
 }
@@ -228,6 +231,18 @@ void advance_outer(size_t) {  // solver.cpp:254-263: memcpy M<-P on the device, 
     swap_host(&ths, &thps);
     swap_host(&ws, &wps);
     check(dyn_advance_outer(g_ctx), "dyn_advance_outer");
+}
+
+// ---- turbulence.h ------------------------------------------------------------------------------------
+// calculate_diff_tend, turbulence.cpp:158-190: coefficients and tendencies from the M buffers in HBM
+void calculate_diff_tend(int il, int ih, int jl, int jh) { CALL(dyn_calculate_diff_tend(g_ctx, il, ih, jl, jh)); }
+// apply_velocity_diffusion / apply_scalar_diffusion / apply_moisture_diffusion, turbulence.cpp:233-292
+void apply_velocity_diffusion(double step, int il, int ih, int jl, int jh) {
+    CALL(dyn_apply_velocity_diffusion(g_ctx, step, il, ih, jl, jh));
+}
+void apply_scalar_diffusion(double step, int il, int ih, int jl, int jh) { CALL(dyn_apply_scalar_diffusion(g_ctx, step, il, ih, jl, jh)); }
+void apply_moisture_diffusion(double step, int il, int ih, int jl, int jh) {
+    CALL(dyn_apply_moisture_diffusion(g_ctx, step, il, ih, jl, jh));
 }
 
 // ---- boundaries.h / damping.h ------------------------------------------------------------------------

@@ -1,5 +1,6 @@
 // Declarations of the reference's host globals that its dynamical-core entry points use
-// (model.h:16-26, 63-91, 220-245; microphysics.h:18-25; stdafx.h:51-68; boundaries.h:4-7; damping.h:2).
+// (model.h:16-26, 63-91, 220-245; microphysics.h:18-25; stdafx.h:51-68; boundaries.h:4-7; damping.h:2;
+// turbulence.h:2).
 // In the reference build these are defined by initializer.cpp; the standalone build defines them in
 // refapi_host.cpp.
 #pragma once
@@ -20,3 +21,4 @@ extern int MICROPHYSICS_OPTION, USE_MICROPHYSICS, USE_ICE, RAIN_FALLOUT, PERIODI
 extern int USE_TURBULENT_STRESS, USE_EXPLICIT_DIFFUSION;
 extern int iwbuffer, iebuffer, jnbuffer, jsbuffer;
 extern int raydampheight;
+extern double *landsea;  // turbulence.cpp:32, filled by initialize_landsea (NX*NY, < 0.5 land)

@@ -77,6 +77,13 @@ struct orc_state {
     cplx *in3d;
     /* sponge coefficients (init_boundaries, boundaries.cpp:44-114) */
     double *north_c, *south_c;
+    /* turbulence parameterization (turbulence.cpp:32-60, 85-147): mixing coefficients, stored
+     * tendencies, the land-sea mask and the saturation flags Kessler leaves (kessler.cpp:129-150) */
+    int turb;
+    double *kmix, *khmix, *ufric, *vfric, *wfric, *tdiff, *qvdiff, *qcdiff, *qrdiff, *landsea;
+    unsigned char *saturated;
+    double *vml2, *vkmax;
+    double kmixh_max, kmixv_max, lH;
 };
 
 #define IDX(i, j, k) ((size_t)(i) * (size_t)s->NYNZ + (size_t)(j) * (size_t)s->NZ + (size_t)(k))
@@ -769,8 +776,15 @@ void orc_apply_bc_micro(orc_state *s) {
 /* ------------------------------------------------------------------------------------------------
  * Kessler warm rain, kessler.cpp:25-213, and precip_rate, microphysics.cpp:143-160
  * ----------------------------------------------------------------------------------------------*/
+static int kessler_point(double dt, double rhou, double tbk, double qbk, double tbvk, double thbar, double qbar, double pbar,
+                         double pi, double *TH, double *QV, double *QC, double *QR, unsigned char *sat);
 int orc_kessler_point(double dt, double rhou, double tbk, double qbk, double tbvk, double thbar, double qbar, double pbar,
                       double pi, double *TH, double *QV, double *QC, double *QR) {
+    return kessler_point(dt, rhou, tbk, qbk, tbvk, thbar, qbar, pbar, pi, TH, QV, QC, QR, NULL);
+}
+/* sat (when not NULL) receives isSaturated of kessler.cpp:129-150, set where USE_TURBULENT_STRESS */
+static int kessler_point(double dt, double rhou, double tbk, double qbk, double tbvk, double thbar, double qbar, double pbar,
+                         double pi, double *TH, double *QV, double *QC, double *QR, unsigned char *sat) {
     const double cpRd = cp / Rd;
     double A, B, C, E, cvent, qvsat, pd, theta, pressure, phi, vapor, total_convert, temperature, dc, de, esl, f;
     int err = 0;
@@ -806,8 +820,15 @@ int orc_kessler_point(double dt, double rhou, double tbk, double qbk, double tbv
     f = 17.67 * (273.15 - 29.65) * Lv / cp;
     phi = pd / (pd - esl) * qvsat * f / pow((temperature - 29.65), 2);
     C = 0.0;
-    if (vapor > qvsat) C = (vapor - qvsat) / (1.0 + phi);
-    else if (vapor < qvsat && *QC > 0) C = -fmin(*QC, (qvsat - vapor) / (1.0 + phi));
+    if (vapor > qvsat) {
+        C = (vapor - qvsat) / (1.0 + phi);
+        if (sat) *sat = 1;
+    } else if (vapor < qvsat && *QC > 0) {
+        C = -fmin(*QC, (qvsat - vapor) / (1.0 + phi));
+        if (sat) *sat = (-C < *QC) ? 1 : 0;
+    } else if (sat) {
+        *sat = (vapor == qvsat) ? 1 : 0;
+    }
     *QC += C;
     *QV -= C;
     dc = (Lv / (cp * pressure)) * C;
@@ -825,8 +846,8 @@ void orc_kessler(orc_state *s, int il, int ih, int jl, int jh) {
         for (int j = jl; j < jh; j++)
             for (int k = 1; k < s->NZ - 1; k++) {
                 size_t n = IDX(i, j, k);
-                err |= orc_kessler_point(s->dt, rhou[k], tb[k], qb[k], tbv[k], s->thbar[n], s->qbar[n], s->pbar[n], s->pis[n], &s->thps[n],
-                                         &s->qvps[n], &s->qcps[n], &s->qrps[n]);
+                err |= kessler_point(s->dt, rhou[k], tb[k], qb[k], tbv[k], s->thbar[n], s->qbar[n], s->pbar[n], s->pis[n], &s->thps[n],
+                                     &s->qvps[n], &s->qcps[n], &s->qrps[n], s->turb ? &s->saturated[n] : NULL);
             }
     if (err) s->error = 1;
     /* precip_rate(il,ih,jl,jh,vts,qrps,accRain), kessler.cpp:210 -> microphysics.cpp:143-160 */
@@ -854,6 +875,305 @@ void orc_rayleigh(orc_state *s, int il, int ih, int jl, int jh, int level) {
                 s->thps[n] -= coefu * s->thps[n];
             }
     }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Turbulence parameterization, turbulence.cpp (USE_TURBULENT_STRESS, no terrain: HTOPO = 0, no surface
+ * heat flux: SURFACE_HEAT_FLUX 0).  Mixing coefficients at the start of each step from the M buffers,
+ * tendencies stored and added during every stage.
+ * ----------------------------------------------------------------------------------------------*/
+static const double cmixv_max = 0.15, cmixh_max = 0.05; /* turbulence.cpp:48-49 */
+static const double one_d_Pr = 3.0, lv_max = 200.0, cs_turb = 0.25; /* :56-58 */
+
+#define KM(i, j, k) s->kmix[IDX(i, j, k)]
+#define KHM(i, j, k) s->khmix[IDX(i, j, k)]
+#define ONE_D_DZ(k) (s->one_d_dz * s->col[C_MU][k])
+#define ONE_D_DZW(k) (s->one_d_dz * s->col[C_MW][k])
+
+/* init_kmix, turbulence.cpp:85-147, with zlevs = ZU = zsu (initialize_serial, driver_serial.cpp:52) */
+static void turb_init(orc_state *s) {
+    s->kmixh_max = cmixh_max * s->dx * s->dx / s->dt;
+    s->kmixv_max = cmixv_max * s->dz * s->dz / s->dt;
+    s->lH = s->dx / 4.0;
+    const double *z = s->col[C_ZSU];
+    for (int k = 1; k < s->NZ; k++) {
+        double deltaZ = z[k] - z[k - 1];
+        s->vml2[k] = deltaZ > lv_max ? lv_max * lv_max : deltaZ * deltaZ;
+        s->vkmax[k] = cmixv_max * s->vml2[k] / s->dt;
+    }
+}
+
+/* the six rate-of-deformation products, get_Kmix(_dry), turbulence.cpp:596-624 */
+static double deformation2(const orc_state *s, int i, int j, int k) {
+    const double *UM = s->ums, *VM = s->vms, *WM = s->wms, *UB = s->ubar, *VB = s->vbar, *WB = s->wbar;
+    const double odx = s->one_d_dx, ody = s->one_d_dy;
+#define A3(F, a, b, c) F[IDX(a, b, c)]
+    double D11 = 0.50 * (A3(UM, i + 1, j, k) + A3(UM, i + 1, j, k - 1) - A3(UM, i - 1, j, k) - A3(UM, i - 1, j, k - 1) +
+                         A3(UB, i + 1, j, k) + A3(UB, i + 1, j, k - 1) - A3(UB, i - 1, j, k) - A3(UB, i - 1, j, k - 1)) * odx;
+    double D22 = 0.50 * (A3(VM, i, j + 1, k) + A3(VM, i, j + 1, k - 1) - A3(VM, i, j - 1, k) - A3(VM, i, j - 1, k - 1) +
+                         A3(VB, i, j + 1, k) + A3(VB, i, j + 1, k - 1) - A3(VB, i, j - 1, k) - A3(VB, i, j - 1, k - 1)) * ody;
+    double D33 = 1.0 * (A3(WM, i, j, k + 1) - A3(WM, i, j, k - 1) + A3(WB, i, j, k + 1) - A3(WB, i, j, k - 1)) * ONE_D_DZW(k);
+    double D12 = 0.5 * ((A3(UM, i, j + 1, k) + A3(UM, i, j + 1, k - 1) - A3(UM, i, j - 1, k) - A3(UM, i, j - 1, k - 1) +
+                         A3(UB, i, j + 1, k) + A3(UB, i, j + 1, k - 1) - A3(UB, i, j - 1, k) - A3(UB, i, j - 1, k - 1)) * ody +
+                        (A3(VM, i + 1, j, k) + A3(VM, i + 1, j, k - 1) - A3(VM, i - 1, j, k) - A3(VM, i - 1, j, k - 1) +
+                         A3(VB, i + 1, j, k) + A3(VB, i + 1, j, k - 1) - A3(VB, i - 1, j, k) - A3(VB, i - 1, j, k - 1)) * odx);
+    double D13 = 0.5 * ((A3(UM, i, j, k) + A3(UM, i + 1, j, k) - A3(UM, i, j, k - 1) - A3(UM, i + 1, j, k - 1) +
+                         A3(UB, i, j, k) + A3(UB, i + 1, j, k) - A3(UB, i, j, k - 1) - A3(UB, i + 1, j, k - 1)) * ONE_D_DZW(k) +
+                        (A3(WM, i + 1, j, k) - A3(WM, i - 1, j, k) + A3(WB, i + 1, j, k) - A3(WB, i - 1, j, k)) * odx);
+    double D23 = 0.5 * ((A3(VM, i, j, k) + A3(VM, i, j + 1, k) - A3(VM, i, j, k - 1) - A3(VM, i, j + 1, k - 1) +
+                         A3(VB, i, j, k) + A3(VB, i, j + 1, k) - A3(VB, i, j, k - 1) - A3(VB, i, j + 1, k - 1)) * ONE_D_DZW(k) +
+                        (A3(WM, i, j + 1, k) - A3(WM, i, j - 1, k) + A3(WB, i, j + 1, k) - A3(WB, i, j - 1, k)) * ody);
+#undef A3
+    return D11 * D11 + D22 * D22 + D33 * D33 + D12 * D12 + D13 * D13 + D23 * D23;
+}
+
+/* the Brunt-Vaisala frequency of get_Kmix_dry (turbulence.cpp:781-791) and get_Kmix (:627-690) */
+static double brunt_vaisala(const orc_state *s, int i, int j, int k) {
+    const double *tb = s->col[C_TB], *qb = s->col[C_QB], *tbv = s->col[C_TBV];
+    const size_t h = IDX(i, j, k), l = IDX(i, j, k - 1);
+    if (!s->moist || !s->saturated[h] || !s->saturated[l]) {
+        double qh = s->moist ? s->qvms[h] + s->qbar[h] + qb[k] : qb[k];
+        double ql = s->moist ? s->qvms[l] + s->qbar[l] + qb[k - 1] : qb[k - 1];
+        double thetaVH = (s->thms[h] + s->thbar[h] + tb[k]) * (1 + 0.61 * qh);
+        double thetaVL = (s->thms[l] + s->thbar[l] + tb[k - 1]) * (1 + 0.61 * ql);
+        double dTheta = thetaVH - thetaVL;
+        return 2.0 * grav / (tbv[k] + tbv[k - 1]) * dTheta * ONE_D_DZW(k);
+    }
+    /* saturated: equivalent potential temperature and the moist-adiabatic lapse (turbulence.cpp:652-690) */
+    const double *rhou = s->col[C_RHOU], *tbw = s->col[C_TBW];
+    const double cpRd = cp / Rd;
+    double thetaVH = s->thms[h] + s->thbar[h] + tb[k];
+    double thetaVL = s->thms[l] + s->thbar[l] + tb[k - 1];
+    double tempH = thetaVH * (s->pbar[h] + s->pis[h] / (cp * tbv[k]));
+    double tempL = thetaVL * (s->pbar[l] + s->pis[l] / (cp * tbv[k - 1]));
+    double temp = 0.5 * (tempH + tempL);
+    double dQL = s->qcms[h] + s->qrms[h] - s->qcms[l] - s->qrms[l] + s->qvms[h] - s->qvms[l] + s->qbar[h] - s->qbar[l] + qb[k] - qb[k - 1];
+    double qv = 0.5 * (s->qvms[h] + s->qvms[l] + s->qbar[h] + s->qbar[l] + qb[k] + qb[k - 1]);
+    double pdH = p0 * pow(s->pbar[h], cpRd) + s->pis[h] * rhou[k];
+    double pdL = p0 * pow(s->pbar[l], cpRd) + s->pis[l] * rhou[k - 1];
+    double eslH = 611.2 * exp(17.67 * (tempH - 273.15) / (tempH - 29.65));
+    double eslL = 611.2 * exp(17.67 * (tempL - 273.15) / (tempL - 29.65));
+    double qvsatH = 0.62197 * eslH / (pdH - eslH);
+    double qvsatL = 0.62197 * eslL / (pdL - eslL);
+    thetaVH = thetaVH * (1 + Lv * qvsatH / (cp * tempH));
+    thetaVL = thetaVL * (1 + Lv * qvsatL / (cp * tempL));
+    double dTheta = thetaVH - thetaVL;
+    double A = (grav / tbw[k]) * (1.0 + Lv * qv / (Rd * temp)) / (1.0 + 0.622 * Lv * Lv * qv / (cp * Rd * temp * temp));
+    return (A * dTheta - grav * dQL) * ONE_D_DZW(k);
+}
+
+/* get_Kmix / get_Kmix_dry, turbulence.cpp:563-735, 737-831: vertical mixing coefficient on w-levels,
+ * horizontal on u-levels (level k-1 set with level k), both bounded above */
+static void turb_kmix(orc_state *s, int il, int ih, int jl, int jh) {
+    const int NZ = s->NZ;
+    const double lH2 = s->lH * s->lH;
+    for (int i = il; i < ih; i++)
+        for (int j = jl; j < jh; j++) {
+            KM(i, j, 1) = 0; /* zero at ground */
+            KM(i, j, NZ - 1) = 0;
+            for (int k = 2; k < NZ - 1; k++) {
+                double S2 = deformation2(s, i, j, k);
+                double bruntv = brunt_vaisala(s, i, j, k);
+                if (S2 - one_d_Pr * bruntv > 0) {
+                    double c = s->vml2[k] * cs_turb * cs_turb;
+                    KM(i, j, k) = c * sqrt(S2 - one_d_Pr * bruntv);
+                    KHM(i, j, k - 1) = 0.5 * (KM(i, j, k - 1) / s->vml2[k - 1] + KM(i, j, k) / s->vml2[k]) * s->lH * s->lH;
+                    if (KM(i, j, k) > s->vkmax[k]) KM(i, j, k) = s->vkmax[k];
+                    if (KHM(i, j, k - 1) > s->kmixh_max) KHM(i, j, k - 1) = s->kmixh_max;
+                } else {
+                    KM(i, j, k) = 0;
+                    KHM(i, j, k - 1) = 0 + 0.5 * KM(i, j, k - 1) / s->vml2[k - 1] * s->lH * s->lH;
+                }
+            }
+        }
+    (void)lH2;
+}
+
+/* components of the rate-of-deformation tensor at slab i (turbulence.cpp:913-933); level 0 of tau_33,
+ * tau_13 and tau_23 is never computed and stays zero (the stress cells are zero-filled) */
+static double tau11(const orc_state *s, int i, int j, int k) { return 2.0 * (s->ums[IDX(i + 1, j, k)] - s->ums[IDX(i, j, k)]) * s->one_d_dx; }
+static double tau22(const orc_state *s, int i, int j, int k) { return 2.0 * (s->vms[IDX(i, j + 1, k)] - s->vms[IDX(i, j, k)]) * s->one_d_dy; }
+static double tau33(const orc_state *s, int i, int j, int k) {
+    return k < 1 ? 0.0 : 2.0 * (s->wms[IDX(i, j, k + 1)] - s->wms[IDX(i, j, k)]) * ONE_D_DZ(k);
+}
+static double tau13(const orc_state *s, int i, int j, int k) {
+    if (k < 1) return 0.0;
+    return (s->wms[IDX(i + 1, j, k + 1)] - s->wms[IDX(i, j, k + 1)]) * s->one_d_dx +
+           (s->ums[IDX(i + 1, j, k + 1)] - s->ums[IDX(i + 1, j, k)]) * ONE_D_DZW(k + 1);
+}
+static double tau23(const orc_state *s, int i, int j, int k) {
+    if (k < 1) return 0.0;
+    return (s->wms[IDX(i, j + 1, k + 1)] - s->wms[IDX(i, j, k + 1)]) * s->one_d_dy +
+           (s->vms[IDX(i, j + 1, k + 1)] - s->vms[IDX(i, j + 1, k)]) * ONE_D_DZW(k + 1);
+}
+static double tau12(const orc_state *s, int i, int j, int k) {
+    return (s->ums[IDX(i + 1, j + 1, k)] - s->ums[IDX(i + 1, j, k)]) * s->one_d_dy +
+           (s->vms[IDX(i + 1, j + 1, k)] - s->vms[IDX(i, j + 1, k)]) * s->one_d_dx;
+}
+
+/* surface stress of the lowest level (turbulence.cpp:885-907): land (landsea < 0.5) a fixed drag
+ * coefficient, ocean one that grows with the wind speed */
+static void surface_stress(const orc_state *s, int i, int j, double *tu, double *tv) {
+    const size_t n = IDX(i, j, 1);
+    double wind_speed = sqrt((s->ums[n] + s->ubar[n]) * (s->ums[n] + s->ubar[n]) + (s->vms[n] + s->vbar[n]) * (s->vms[n] + s->vbar[n]));
+    double wind_speed_base = sqrt(s->ubar[n] * s->ubar[n] + s->vbar[n] * s->vbar[n]);
+    if (s->landsea[(size_t)i * s->NY + j] < 0.5) {
+        *tu = 0.003 * (wind_speed * s->ums[n] + (wind_speed - wind_speed_base) * s->ubar[n]);
+        *tv = 0.003 * (wind_speed * s->vms[n] + (wind_speed - wind_speed_base) * s->vbar[n]);
+    } else {
+        double drag_coef = (1.1e-3 + 4.0e-5 * wind_speed);
+        *tu = drag_coef * (wind_speed * s->ums[n] + (wind_speed - wind_speed_base) * s->ubar[n]);
+        *tv = drag_coef * (wind_speed * s->vms[n] + (wind_speed - wind_speed_base) * s->vbar[n]);
+    }
+}
+
+/* turbulent_diffusion_velocity, turbulence.cpp:839-1020: friction tendencies of u, v, w over
+ * [il,ih) x [jl,jh) x [1,NZ-1); the stress of slab i-1 is the west stress of slab i */
+static void turb_velocity(orc_state *s, int il, int ih, int jl, int jh) {
+    const int NZ = s->NZ;
+    const double odx = s->one_d_dx, ody = s->one_d_dy;
+    for (int i = il; i < ih; i++)
+        for (int j = jl; j < jh; j++) {
+            double tau_u_zl, tau_v_zl;
+            surface_stress(s, i, j, &tau_u_zl, &tau_v_zl);
+            for (int k = 1; k < NZ - 1; k++) {
+                double uK11H_h = KHM(i, j, k);
+                double uK11H_l = KHM(i - 1, j, k);
+                double uK12H_h = 0.25 * (KHM(i, j, k) + KHM(i - 1, j, k) + KHM(i - 1, j + 1, k) + KHM(i, j + 1, k));
+                double uK12H_l = 0.25 * (KHM(i, j - 1, k) + KHM(i - 1, j - 1, k) + KHM(i - 1, j, k) + KHM(i, j, k));
+                double uK13V_h = 0.50 * (KM(i, j, k + 1) + KM(i - 1, j, k + 1));
+                double vK12H_h = 0.25 * (KHM(i, j - 1, k) + KHM(i, j, k) + KHM(i + 1, j - 1, k) + KHM(i + 1, j, k));
+                double vK12H_l = uK12H_l;
+                double vK22H_h = KHM(i, j, k);
+                double vK22H_l = KHM(i, j - 1, k);
+                double vK23V_h = 0.5 * (KM(i, j, k + 1) + KM(i, j - 1, k + 1));
+                double wK13H_h = 0.25 * (KHM(i, j, k) + KHM(i, j, k - 1) + KHM(i + 1, j, k) + KHM(i + 1, j, k - 1));
+                double wK13H_l = 0.25 * (KHM(i, j, k) + KHM(i, j, k - 1) + KHM(i - 1, j, k) + KHM(i - 1, j, k - 1));
+                double wK23H_h = 0.25 * (KHM(i, j, k) + KHM(i, j, k - 1) + KHM(i, j + 1, k) + KHM(i, j + 1, k - 1));
+                double wK23H_l = 0.25 * (KHM(i, j, k) + KHM(i, j, k - 1) + KHM(i, j - 1, k) + KHM(i, j - 1, k - 1));
+                double wK33V_h = 0.50 * (KM(i, j, k) + KM(i, j, k + 1));
+                double wK33V_l = 0.50 * (KM(i, j, k) + KM(i, j, k - 1));
+                double ufric = (uK11H_h * tau11(s, i, j, k) - uK11H_l * tau11(s, i - 1, j, k)) * odx +
+                               (uK12H_h * tau12(s, i - 1, j, k) - uK12H_l * tau12(s, i - 1, j - 1, k)) * ody +
+                               (uK13V_h * tau13(s, i - 1, j, k) - tau_u_zl) * ONE_D_DZ(k);
+                double vfric = (vK12H_h * tau12(s, i, j - 1, k) - vK12H_l * tau12(s, i - 1, j - 1, k)) * odx +
+                               (vK22H_h * tau22(s, i, j, k) - vK22H_l * tau22(s, i, j - 1, k)) * ody +
+                               (vK23V_h * tau23(s, i, j - 1, k) - tau_v_zl) * ONE_D_DZ(k);
+                double wfric = (wK13H_h * tau13(s, i, j, k - 1) - wK13H_l * tau13(s, i - 1, j, k - 1)) * odx +
+                               (wK23H_h * tau23(s, i, j, k - 1) - wK23H_l * tau23(s, i, j - 1, k - 1)) * ody +
+                               (wK33V_h * tau33(s, i, j, k) - wK33V_l * tau33(s, i, j, k - 1)) * ONE_D_DZW(k);
+                s->ufric[IDX(i, j, k)] = ufric;
+                s->vfric[IDX(i, j, k)] = vfric;
+                s->wfric[IDX(i, j, k)] = wfric;
+                tau_u_zl = uK13V_h * tau13(s, i - 1, j, k);
+                tau_v_zl = vK23V_h * tau23(s, i, j - 1, k);
+            }
+        }
+}
+
+/* the diffusion tendency of one scalar at (i,j,k) (turbulence.cpp:1128-1174, 1226-1266); the bottom
+ * flux is the top flux of the level below, zero at the ground (no surface heat flux) */
+static double scalar_diffusion(const orc_state *s, const double *S, int i, int j, int k, double tau_l) {
+    double tau_h = -one_d_Pr * KM(i, j, k + 1) * (S[IDX(i, j, k + 1)] - S[IDX(i, j, k)]) * ONE_D_DZW(k + 1);
+    double KH = 0.5 * (one_d_Pr * KHM(i, j + 1, k) + one_d_Pr * KHM(i, j, k));
+    double KL = 0.5 * (one_d_Pr * KHM(i, j, k) + one_d_Pr * KHM(i, j - 1, k));
+    double t22h = -KH * (S[IDX(i, j + 1, k)] - S[IDX(i, j, k)]) * s->one_d_dy;
+    double t22l = -KL * (S[IDX(i, j, k)] - S[IDX(i, j - 1, k)]) * s->one_d_dy;
+    KH = 0.5 * (one_d_Pr * KHM(i + 1, j, k) + one_d_Pr * KHM(i, j, k));
+    KL = 0.5 * (one_d_Pr * KHM(i, j, k) + one_d_Pr * KHM(i - 1, j, k));
+    double t11h = -KH * (S[IDX(i + 1, j, k)] - S[IDX(i, j, k)]) * s->one_d_dx;
+    double t11l = -KL * (S[IDX(i, j, k)] - S[IDX(i - 1, j, k)]) * s->one_d_dx;
+    return -((t11h - t11l) * s->one_d_dx + (t22h - t22l) * s->one_d_dy + (tau_h - tau_l) * ONE_D_DZ(k));
+}
+
+/* turbulent_diffusion_theta / turbulent_diffusion_scalars, turbulence.cpp:1027-1188, 1192-1276 */
+static void turb_scalars(orc_state *s, int il, int ih, int jl, int jh) {
+    const int NZ = s->NZ;
+    const double *F[4] = {s->thms, s->qvms, s->qcms, s->qrms};
+    double *T[4] = {s->tdiff, s->qvdiff, s->qcdiff, s->qrdiff};
+    const int nf = s->moist ? 4 : 1;
+    for (int i = il; i < ih; i++)
+        for (int j = jl; j < jh; j++)
+            for (int f = 0; f < nf; f++) {
+                double tau_l = 0.0;
+                for (int k = 1; k < NZ - 1; k++) {
+                    T[f][IDX(i, j, k)] = scalar_diffusion(s, F[f], i, j, k, tau_l);
+                    tau_l = -one_d_Pr * KM(i, j, k + 1) * (F[f][IDX(i, j, k + 1)] - F[f][IDX(i, j, k)]) * ONE_D_DZW(k + 1);
+                }
+            }
+}
+
+void orc_calculate_diff_tend(orc_state *s, int il, int ih, int jl, int jh) {
+    if (!s->turb) return;
+    turb_kmix(s, il, ih, jl, jh);
+    turb_velocity(s, il, ih, jl, jh);
+    turb_scalars(s, il, ih, jl, jh);
+}
+
+/* apply_velocity_diffusion, turbulence.cpp:233-258: w from level 2 */
+void orc_apply_velocity_diffusion(orc_state *s, double step, int il, int ih, int jl, int jh) {
+    double deltaT = step * s->dt;
+    for (int i = il; i < ih; i++)
+        for (int j = jl; j < jh; j++) {
+            size_t ind = IDX(i, j, 1);
+            s->ups[ind] += s->ufric[ind] * deltaT;
+            s->vps[ind] += s->vfric[ind] * deltaT;
+            for (int k = 2; k < s->NZ - 1; k++) {
+                ind = IDX(i, j, k);
+                s->ups[ind] += s->ufric[ind] * deltaT;
+                s->vps[ind] += s->vfric[ind] * deltaT;
+                s->wps[ind] += s->wfric[ind] * deltaT;
+            }
+        }
+}
+
+/* apply_scalar_diffusion, turbulence.cpp:263-273 */
+void orc_apply_scalar_diffusion(orc_state *s, double step, int il, int ih, int jl, int jh) {
+    for (int i = il; i < ih; i++)
+        for (int j = jl; j < jh; j++)
+            for (int k = 1; k < s->NZ - 1; k++) s->thps[IDX(i, j, k)] += s->tdiff[IDX(i, j, k)] * step * s->dt;
+}
+
+/* apply_moisture_diffusion, turbulence.cpp:278-292 */
+void orc_apply_moisture_diffusion(orc_state *s, double step, int il, int ih, int jl, int jh) {
+    double deltaT = step * s->dt;
+    for (int i = il; i < ih; i++)
+        for (int j = jl; j < jh; j++)
+            for (int k = 1; k < s->NZ - 1; k++) {
+                size_t n = IDX(i, j, k);
+                s->qvps[n] += s->qvdiff[n] * deltaT;
+                s->qcps[n] += s->qcdiff[n] * deltaT;
+                s->qrps[n] += s->qrdiff[n] * deltaT;
+            }
+}
+
+/* USE_TURBULENT_STRESS with the land-sea mask (NULL: all ocean, initialize_landsea without a file,
+ * turbulence.cpp:370-375) */
+void orc_set_turbulence(orc_state *s, int on, const double *landsea) {
+    s->turb = on;
+    if (!on) return;
+    size_t size = (size_t)s->NX * s->NYNZ;
+    double **bufs[] = {&s->kmix, &s->khmix, &s->ufric, &s->vfric, &s->wfric, &s->tdiff, &s->qvdiff, &s->qcdiff, &s->qrdiff};
+    for (size_t b = 0; b < sizeof(bufs) / sizeof(bufs[0]); b++)
+        if (!*bufs[b]) *bufs[b] = (double *)calloc(size, sizeof(double));
+    if (!s->saturated) s->saturated = (unsigned char *)calloc(size, 1);
+    if (!s->landsea) s->landsea = (double *)calloc((size_t)s->NX * s->NY, sizeof(double));
+    for (size_t n = 0; n < (size_t)s->NX * s->NY; n++) s->landsea[n] = landsea ? landsea[n] : 1.0;
+    if (!s->vml2) s->vml2 = (double *)calloc(s->NZ, sizeof(double));
+    if (!s->vkmax) s->vkmax = (double *)calloc(s->NZ, sizeof(double));
+    turb_init(s);
+}
+
+/* the turbulence arrays, for tests: 0 Kmix, 1 KHmix, 2-4 u/v/w friction, 5-8 theta/qv/qc/qr diffusion,
+ * 9 Kessler's saturation flag (0 or 1) */
+void orc_get_turbulence(const orc_state *s, int which, double *out) {
+    const double *src[] = {s->kmix, s->khmix, s->ufric, s->vfric, s->wfric, s->tdiff, s->qvdiff, s->qcdiff, s->qrdiff};
+    if (which == 9 && s->saturated) {
+        for (size_t n = 0; n < (size_t)s->NX * s->NYNZ; n++) out[n] = s->saturated[n];
+        return;
+    }
+    if (which < 0 || which > 8 || !src[which]) return;
+    memcpy(out, src[which], sizeof(double) * (size_t)s->NX * s->NYNZ);
 }
 
 /* ------------------------------------------------------------------------------------------------
@@ -889,21 +1209,25 @@ void orc_micro_advance_outer(orc_state *s) {
 }
 
 /* ------------------------------------------------------------------------------------------------
- * s_integrate_rk3, driver_serial.cpp:178-286 (no diffusion, no turbulence, no terrain; the
- * prescribed heating never fires, Heating.h:3)
+ * s_integrate_rk3, driver_serial.cpp:178-286 (no explicit diffusion, no terrain; the prescribed heating
+ * never fires, Heating.h:3; turbulence when orc_set_turbulence turned it on)
  * ----------------------------------------------------------------------------------------------*/
 static void integrate_rk3(orc_state *s) {
     const double steps[] = {1. / 3., 0.5, 1.0};
     const int NX = s->NX, NY = s->NY;
+    if (s->turb) orc_calculate_diff_tend(s, 1, NX - 1, 1, NY - 1); /* driver_serial.cpp:194 */
     for (int st = 0; st < 3; st++) {
         double step = steps[st];
         /* integrate_non_hydro(step,0,NX,0,NY), solver.cpp:87-138 */
         orc_advect_uvw(s, step, 3, NX - 3, 3, NY - 3);
+        if (s->turb) orc_apply_velocity_diffusion(s, step, 3, NX - 3, 3, NY - 3); /* solver.cpp:104 */
         orc_solve_pressure(s, step);
         orc_pgf(s, step, 1, NX - 1, 1, NY - 1);
         orc_advect_theta(s, step, 3, NX - 3, 3, NY - 3);
+        if (s->turb) orc_apply_scalar_diffusion(s, step, 1, NX - 1, 1, NY - 1); /* driver_serial.cpp:218 */
         if (s->moist) {
             orc_advect_moisture(s, step, 3, NX - 3, 3, NY - 3);
+            if (s->turb) orc_apply_moisture_diffusion(s, step, 1, NX - 1, 1, NY - 1); /* driver_serial.cpp:231 */
             orc_zero_moisture(s, 1, NX - 1, 1, NY - 1);
         }
         orc_apply_bc(s);
@@ -964,9 +1288,12 @@ void orc_destroy(orc_state *s) {
     double *bufs[] = {s->us, s->ups, s->ums, s->vs, s->vps, s->vms, s->ws, s->wps, s->wms, s->ths, s->thps, s->thms,
                       s->qvs, s->qvps, s->qvms, s->qcs, s->qcps, s->qcms, s->qrs, s->qrps, s->qrms, s->pis, s->vts,
                       s->ubar, s->vbar, s->wbar, s->thbar, s->qbar, s->pbar, s->accrain, s->north_c, s->south_c,
-                      s->kwave, s->lwave, s->acoef, s->ccoef, s->bcoef, s->rhs};
+                      s->kwave, s->lwave, s->acoef, s->ccoef, s->bcoef, s->rhs,
+                      s->kmix, s->khmix, s->ufric, s->vfric, s->wfric, s->tdiff, s->qvdiff, s->qcdiff, s->qrdiff,
+                      s->landsea, s->vml2, s->vkmax};
     for (size_t b = 0; b < sizeof(bufs) / sizeof(bufs[0]); b++) free(bufs[b]);
     free(s->in3d);
+    free(s->saturated);
     free(s);
 }
 

@@ -82,6 +82,17 @@ void orc_advance_outer(orc_state *s);
 void orc_micro_advance_inner(orc_state *s);
 void orc_micro_advance_outer(orc_state *s);
 
+/* turbulence parameterization, turbulence.cpp: USE_TURBULENT_STRESS on or off, with the land-sea mask
+ * landsea[NX*NY] (NULL: all ocean, as initialize_landsea sets it without a file, turbulence.cpp:370-375);
+ * when on, orc_step calls calculate_diff_tend and the apply_* functions where s_integrate_rk3 does */
+void orc_set_turbulence(orc_state *s, int on, const double *landsea);
+void orc_calculate_diff_tend(orc_state *s, int il, int ih, int jl, int jh);
+void orc_apply_velocity_diffusion(orc_state *s, double step, int il, int ih, int jl, int jh);
+void orc_apply_scalar_diffusion(orc_state *s, double step, int il, int ih, int jl, int jh);
+void orc_apply_moisture_diffusion(orc_state *s, double step, int il, int ih, int jl, int jh);
+/* 0 Kmix, 1 KHmix, 2-4 u/v/w friction, 5-8 theta/qv/qc/qr diffusion tendencies, [NX][NY][NZ] */
+void orc_get_turbulence(const orc_state *s, int which, double *out);
+
 /* ---- pieces exposed so tests can pin them against mathematics ---- */
 /* in-place complex DFT of length n over interleaved (re,im) data; sign -1 forward, +1 backward,
  * unnormalised (the FFTW convention used at pressure.cpp:747-748) */

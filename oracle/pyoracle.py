@@ -78,6 +78,11 @@ def lib(variant: str = "parity"):
     L.orc_kessler_point.restype = ctypes.c_int
     L.orc_fall_speed.argtypes = [ctypes.c_double] * 3
     L.orc_fall_speed.restype = ctypes.c_double
+    L.orc_set_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.orc_calculate_diff_tend.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
+    for fn in ("orc_apply_velocity_diffusion", "orc_apply_scalar_diffusion", "orc_apply_moisture_diffusion"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4
+    L.orc_get_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
     _LIBS[variant] = L
     return L
 
@@ -136,6 +141,18 @@ class Oracle:
 
     def step(self, n=1):
         self.L.orc_step(self.h, n)
+
+    def set_turbulence(self, on=True, landsea=None):
+        ls = None if landsea is None else np.ascontiguousarray(landsea, dtype=np.float64)
+        self._landsea = ls
+        self.L.orc_set_turbulence(self.h, int(on), None if ls is None else _ptr(ls))
+
+    def get_turbulence(self, which):
+        """0 Kmix, 1 KHmix, 2-4 u/v/w friction, 5-8 theta/qv/qc/qr diffusion tendencies, 9 saturation flag."""
+        c = self.case
+        out = np.zeros((c.nx, c.ny, c.nz))
+        self.L.orc_get_turbulence(self.h, which, _ptr(out))
+        return out
 
     @property
     def error(self):

@@ -16,22 +16,29 @@ extern int iebuffer, iwbuffer, jnbuffer, jsbuffer;
 int load_input(const char *path);
 int write_output(const char *path);
 
+extern double *zu;
+
 void initialize_serial() {  // driver_serial.cpp:44-77, with the initial state already in the globals
+    if (USE_TURBULENT_STRESS) init_kmix(NX, NY, NZ, zu);
     if (USE_MICROPHYSICS) init_microphysics(NX, NY);
+    initialize_landsea(nullptr);
     initialize_pressure_solver();
     init_boundaries(iebuffer, iwbuffer, jnbuffer, jsbuffer, -1);
     init_damping(NX, NY, NZ);
 }
 
-void s_integrate_rk3() {  // driver_serial.cpp:178-286 (no diffusion, no turbulence, no terrain)
+void s_integrate_rk3() {  // driver_serial.cpp:178-286 (no explicit diffusion, no terrain)
     double steps[] = {1. / 3., 0.5, 1.0};
     int size = NX * NY * NZ;
+    if (USE_TURBULENT_STRESS) calculate_diff_tend(1, NX - 1, 1, NY - 1);
     for (int s = 0; s < 3; s++) {
         compute_sign_cells(0, NX, 0, NY);
         integrate_non_hydro(steps[s], 0, NX, 0, NY);
         advect_theta(steps[s], 3, NX - 3, 3, NY - 3);
+        if (USE_TURBULENT_STRESS) apply_scalar_diffusion(steps[s], 1, NX - 1, 1, NY - 1);
         if (USE_MICROPHYSICS) {
             advect_microphysics_cell(steps[s], 3, NX - 3, 3, NY - 3);
+            if (USE_TURBULENT_STRESS) apply_moisture_diffusion(steps[s], 1, NX - 1, 1, NY - 1);
             zero_moisture(1, NX - 1, 1, NY - 1, size);
         }
         apply_boundary_condition(0);
@@ -70,7 +77,7 @@ int main(int argc, char **argv) {
         fprintf(stderr, "drop-in error: %s\n", dyn_refapi_error());
         return 4;
     }
-    if (init_calls != (USE_MICROPHYSICS ? 3 : 2)) {
+    if (init_calls != 3 + (USE_MICROPHYSICS ? 1 : 0) + (USE_TURBULENT_STRESS ? 1 : 0)) {
         fprintf(stderr, "kept initialisers ran %d times\n", init_calls);
         return 5;
     }

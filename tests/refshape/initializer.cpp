@@ -25,6 +25,7 @@ int USE_TURBULENT_STRESS = 0, USE_EXPLICIT_DIFFUSION = 0;
 int raydampheight = 0;
 int bigcounter = 0;
 int init_calls = 0;
+double *landsea_input = nullptr;
 extern int jnbuffer, jsbuffer;
 
 double *field_ptr(int field, int role) {
@@ -62,13 +63,14 @@ static void rd(void *p, size_t bytes, FILE *fp) {
     }
 }
 
-// File layout: int32 {nx, ny, nz, microphysics, raydampheight, jn, js}, float64 {dx, dz, dt}, the 16
-// columns (f has NY values, the rest NZ), then for every field in dyn_field order its roles (three for
-// a prognostic, one otherwise) as NX*NY*NZ doubles (accrain NX*NY).
+// File layout: int32 {nx, ny, nz, microphysics, raydampheight, jn, js, turbulence, mask}, float64 {dx,
+// dz, dt}, the 16 columns (f has NY values, the rest NZ), the land-sea mask (NX*NY doubles) when mask
+// is 1, then for every field in dyn_field order its roles (three for a prognostic, one otherwise) as
+// NX*NY*NZ doubles (accrain NX*NY).
 int load_input(const char *path) {
     FILE *fp = fopen(path, "rb");
     if (!fp) return 1;
-    int hdr[7];
+    int hdr[9];
     double d[3];
     rd(hdr, sizeof hdr, fp);
     rd(d, sizeof d, fp);
@@ -81,6 +83,7 @@ int load_input(const char *path) {
     raydampheight = hdr[4];
     jnbuffer = hdr[5];
     jsbuffer = hdr[6];
+    USE_TURBULENT_STRESS = hdr[7];
     dx = dy = d[0];
     dz = d[1];
     dt = d[2];
@@ -100,6 +103,10 @@ int load_input(const char *path) {
         size_t len = c == DYN_COL_F ? (size_t)NY : (size_t)NZ;
         *column_slot(c) = (double *)calloc(len, sizeof(double));
         rd(*column_slot(c), len * sizeof(double), fp);
+    }
+    if (hdr[8]) {
+        landsea_input = (double *)malloc((size_t)NX * NY * sizeof(double));
+        rd(landsea_input, (size_t)NX * NY * sizeof(double), fp);
     }
     for (int fld = 0; fld < DYN_FIELD_COUNT; fld++) {
         int roles = fld <= DYN_QR ? 3 : 1;

@@ -23,5 +23,7 @@ void init_boundaries(int eastLength, int westLength, int northLength, int southL
 void init_damping(int ni, int nj, int nk);                                                         // damping.h:14
 void init_microphysics(int nx, int ny);                                                            // microphysics.h:66
 void switch_array(double **var1, double **var2);                                                   // solver.h:7
+void init_kmix(int nx, int ny, int nz, double *zlevs);                                             // turbulence.h:7
+void initialize_landsea(const char *name);                                                         // turbulence.h:6
 extern int bigcounter;
 extern int init_calls;  // counts the kept initialisers that ran

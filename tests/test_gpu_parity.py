@@ -111,12 +111,15 @@ def test_supercell_reduced_grid_30_steps_fp64():
         assert e <= 1e-11, (ids.FIELD_NAMES[f], e)
 
 
-def _write_refshape_input(c, path):
+def _write_refshape_input(c, path, turbulence=False, landsea=None):
     with open(path, "wb") as fh:
-        np.array([c.nx, c.ny, c.nz, c.microphysics, c.rayleigh_level, c.sponge, c.sponge], dtype=np.int32).tofile(fh)
+        np.array([c.nx, c.ny, c.nz, c.microphysics, c.rayleigh_level, c.sponge, c.sponge, int(turbulence),
+                  int(landsea is not None)], dtype=np.int32).tofile(fh)
         np.array([c.dx, c.dz, c.dt], dtype=np.float64).tofile(fh)
         for name in ids.COLUMN_NAMES:
             np.ascontiguousarray(c.columns[name], dtype=np.float64).tofile(fh)
+        if landsea is not None:
+            np.ascontiguousarray(landsea, dtype=np.float64).tofile(fh)
         for f, name in enumerate(ids.FIELD_NAMES):
             arr = c.fields.get(name)
             if arr is None:

@@ -1,7 +1,7 @@
 """The drop-in link recipe, checked on an executable shaped like the reference's serial build (CPU).
 
 tests/refshape/ holds one stand-in source per reference object (initializer, driver_serial, pressure,
-advection, fluxes, kessler, microphysics, boundaries, damping, solver), each defining what that object
+advection, fluxes, kessler, microphysics, boundaries, damping, solver, turbulence), each defining what that object
 defines; the entry points the drop-in replaces stop the program if they ever run.  The Makefile
 localizes the replaced symbols in those objects with the list libdyncore_refapi's build writes, and
 links them against the library (refshape_driver); refshape_control links the same objects without the
@@ -20,16 +20,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 OUT = os.path.join(ROOT, "tests", "refshape", "_build")
 DRIVER = os.path.join(OUT, "refshape_driver")
 CONTROL = os.path.join(OUT, "refshape_control")
-HOST_OBJECTS = ["pressure", "advection", "fluxes", "kessler", "microphysics", "boundaries", "damping", "solver"]
+HOST_OBJECTS = ["pressure", "advection", "fluxes", "kessler", "microphysics", "boundaries", "damping", "solver", "turbulence"]
 # entry points the reference's time loop calls, one or more per replaced object
 MUST_BIND = ["apply_boundary_condition", "apply_boundary_condition_microphysics", "rayleigh_damping", "zero_moisture",
              "run_microphysics", "microphysics_advance", "microphysics_advance_inner", "solve_pressure",
              "initialize_pressure_solver", "integrate_non_hydro", "advect_theta", "advect_microphysics_cell",
-             "advance_inner", "advance_outer", "compute_sign_cells"]
+             "advance_inner", "advance_outer", "compute_sign_cells", "calculate_diff_tend", "apply_velocity_diffusion",
+             "apply_scalar_diffusion", "apply_moisture_diffusion"]
 # functions of the replaced objects the library leaves to the reference
 KEPT = {"boundaries": ["init_boundaries"], "damping": ["init_damping"], "kessler": ["calculate_rain_fall_speed_kessler"],
         "microphysics": ["init_microphysics", "precip_rate", "calculate_eulerian_fall_speed_rain"],
-        "solver": ["switch_array"], "fluxes": ["initialize_flux_cells"]}
+        "solver": ["switch_array"], "fluxes": ["initialize_flux_cells"], "turbulence": ["init_kmix", "initialize_landsea"]}
 
 
 def _demangled(path, *flags):
@@ -98,7 +99,7 @@ def test_entry_points_bind_to_the_library(built):
         got = b.get(mangled[name], [])
         assert got and all(t == "libdyncore_refapi.so" for _, t in got), (name, got)
     # the library's references to the host globals bind to the executable's definitions
-    for g in ("us", "ups", "NX", "MICROPHYSICS_OPTION", "jnbuffer", "raydampheight"):
+    for g in ("us", "ups", "NX", "MICROPHYSICS_OPTION", "jnbuffer", "raydampheight", "landsea", "USE_TURBULENT_STRESS"):
         got = b.get(g, [])
         assert any(src == "libdyncore_refapi.so" and t == "refshape_driver" for src, t in got), (g, got)
 
