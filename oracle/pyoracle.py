@@ -30,13 +30,27 @@ def build():
 
 
 def lib(variant: str = "parity"):
-    name = {"parity": "liboracle.so", "omp": "liboracle_omp.so", "tlm": "liboracle_tlm.so"}[variant]
+    name = {"parity": "liboracle.so", "omp": "liboracle_omp.so", "tlm": "liboracle_tlm.so", "adj": "liboracle_adj.so"}[variant]
     if variant in _LIBS:
         return _LIBS[variant]
     path = os.path.join(_HERE, "_build", name)
     if not os.path.exists(path):
         raise RuntimeError(f"oracle library {path} is not built (run `make -C oracle`)")
     L = ctypes.CDLL(path)
+    if variant == "adj":
+        L.orc_adj_create.restype = ctypes.c_void_p
+        L.orc_adj_create.argtypes = [ctypes.POINTER(OrcGrid)]
+        L.orc_adj_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_adj_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_adj_error.argtypes = [ctypes.c_void_p]
+        L.orc_adj_set_column.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_adj_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_adj_set_initial.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_adj_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_adj_gradient.argtypes = [ctypes.c_void_p] + [_dp] * 8
+        L.orc_adj_tape_size.restype = ctypes.c_longlong
+        _LIBS[variant] = L
+        return L
     if variant == "tlm":
         L.orc_tlm_create.restype = ctypes.c_void_p
         L.orc_tlm_create.argtypes = [ctypes.POINTER(OrcGrid)]
@@ -164,6 +178,68 @@ class Oracle:
         fn = getattr(self.L, "orc_" + name)
         h = self.h
         return lambda *a: fn(h, *a)
+
+
+class Adjoint:
+    """The oracle's discrete adjoint (liboracle_adj.so): the oracle's own arithmetic recorded on a tape and
+    swept backwards.  set_initial(f, x0) makes every element of x0 an input (set into M, current and
+    forecast, as dyn_adjoint_backward defines an initial field); gradient(seeds) returns dJ/dx0 of
+    J = sum(seed * final current field) for u, v, w, theta.  One instance at a time per process (the tape
+    is shared)."""
+
+    def __init__(self, case, initial=None):
+        self.L = lib("adj")
+        self.case = case
+        from model_amd import ids
+        self.ids = ids
+        g = OrcGrid(case.nx, case.ny, case.nz, case.dx, case.dz, case.dt, case.microphysics,
+                    case.rayleigh_level, case.sponge, case.sponge)
+        self.h = self.L.orc_adj_create(ctypes.byref(g))
+        for name, arr in case.columns.items():
+            v = np.ascontiguousarray(arr, dtype=np.float64)
+            self.L.orc_adj_set_column(self.h, ids.COLUMN_ID[name], _ptr(v))
+        for name, arr in case.fields.items():
+            f = ids.FIELD_ID[name]
+            v = np.ascontiguousarray(arr, dtype=np.float64)
+            for role in ((ids.ROLE_M, ids.ROLE_CUR, ids.ROLE_P) if f in ids.PROGNOSTIC else (0,)):
+                self.L.orc_adj_set_field(self.h, f, role, _ptr(v))
+        if initial is not None:
+            for f, x0 in zip((ids.U, ids.V, ids.W, ids.TH), initial):
+                self.set_initial(f, x0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.orc_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_initial(self, f, value):
+        v = np.ascontiguousarray(value, dtype=np.float64)
+        assert self.L.orc_adj_set_initial(self.h, f, _ptr(v)) == 0
+
+    def step(self, n=1):
+        self.L.orc_adj_step(self.h, n)
+
+    def get_field(self, f, role=1):
+        c = self.case
+        out = np.empty((c.nx, c.ny, c.nz))
+        self.L.orc_adj_get_field(self.h, f, role, _ptr(out))
+        return out
+
+    @property
+    def tape_size(self):
+        return self.L.orc_adj_tape_size()
+
+    def gradient(self, seeds):
+        """seeds: [u, v, w, theta] arrays or None; returns the four gradients dJ/dx0."""
+        c = self.case
+        sp = [None if sd is None else np.ascontiguousarray(sd, dtype=np.float64) for sd in seeds]
+        grads = [np.zeros((c.nx, c.ny, c.nz)) for _ in range(4)]
+        rc = self.L.orc_adj_gradient(self.h, *[None if sd is None else _ptr(sd) for sd in sp], *[_ptr(g) for g in grads])
+        assert rc == 0
+        return grads
 
 
 class Tangent:
