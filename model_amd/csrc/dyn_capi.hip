@@ -964,7 +964,8 @@ int dyn_step_fused(dyn_ctx *ctx, int nsteps) {
     CHECK_CTX(c);
     CHECK_COLS(c);
     if (!fused_supported(c)) {
-        set_error("dyn_step_fused: the fused stage kernels need NZ <= 64 or 66 <= NZ <= 128, and fields below 4 GiB");
+        set_error(c->turb.on ? "dyn_step_fused: the fused stage kernels carry no turbulence (dyn_step runs the closure)"
+                             : "dyn_step_fused: the fused stage kernels need NZ <= 64 or 66 <= NZ <= 128, and fields below 4 GiB");
         return DYN_E_UNSUPPORTED;
     }
     if (c->g.rayleigh_level < c->g.nz) {

@@ -173,7 +173,7 @@ def test_refusals_and_switching_off():
     g.set_turbulence(True)
     with pytest.raises(DynError) as e:
         g.step_fused(1)
-    assert e.value.code == -2
+    assert e.value.code == -2 and "turbulence" in str(e.value)
     with pytest.raises(DynError) as e:
         g.adjoint_record(2, 0)
     assert e.value.code == -2
