@@ -159,6 +159,12 @@ def roofline(rows, elt, moist):
             "points_per_launch": int(per_launch_units)}
 
 
+# timing-record name (dyn_timing_report) -> base name of the kernel the record times
+KERNEL_SYMBOL = {
+    "adv_scalars": "k_adv_scalars", "adv_momentum": "k_adv_momentum", "p1_div_rfft_x": "k_div_rfft_x", "p2_ypass": "k_ypass",
+    "p5_irfft_x_out": "k_irfft_x_out", "post_uvw": "k_post_uvw", "post_scalars": "k_post_scalars_shell", "kessler": "k_kessler",
+    "precip": "k_precip", "advance_copy": "k_copy_fields",
+}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_supercell256.json")
 CSRC = os.path.join(ROOT, "model_amd", "csrc")
 
