@@ -40,6 +40,33 @@ __device__ __forceinline__ T i3(T pm1, T p0, T p1, T p2, T sign) {
 template <typename T>
 __device__ __forceinline__ T i2(T p0, T p1) { return T(0.5) * (p1 + p0); }
 
+// Neighbouring levels of a column sit in neighbouring lanes.  Level k+1 and level k-1 come by the
+// wavefront shifts of DPP (wave_shl:1 and wave_shr:1, a modifier of a VALU move: no LDS round trip); a
+// lane whose source lies outside the wavefront keeps its own value.  That is the clamp the callers want at
+// level 0 and at the last lane, and the wavefront boundary of a two-wavefront column is overwritten from
+// LDS by the callers.  Lanes past NZ-1 hold the values loaded at level NZ-1 (clamped loads) or values no
+// active lane reads.  A fixed level comes by readlane (lane index uniform).
+__device__ __forceinline__ int dpp_next(int x) { return __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_prev(int x) { return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_next(float x) { return __int_as_float(dpp_next(__float_as_int(x))); }
+__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(dpp_prev(__float_as_int(x))); }
+__device__ __forceinline__ float lane_read(float x, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane)); }
+__device__ __forceinline__ double lane_next(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)dpp_next((int)b), hi = (unsigned)dpp_next((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double lane_prev(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)dpp_prev((int)b), hi = (unsigned)dpp_prev((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double lane_read(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane), hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Element offsets are 32-bit (the host checks that a field is below 4 GiB): the base pointer stays in
 // SGPRs and one VGPR byte offset serves every field at the same point.
 template <typename T>
@@ -304,10 +331,10 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
         }
         if (k == 0) tf[0] = tf[1] = T(0);
         // bottom face of level k = top face of level k-1; buoyancy of level k-1 (advection.cpp:42-58)
-        T bu = __shfl(tf[0], km1), bv = __shfl(tf[1], km1);
+        T bu = lane_prev(tf[0]), bv = lane_prev(tf[1]);
         T bk = cb[0] / T(c.tbv[kc]);
         if (MOIST) bk = cb[0] / T(c.tbv[kc]) + T(0.61) * cb[1] - cb[2] - cb[3];
-        T bkm = __shfl(bk, km1);
+        T bkm = lane_prev(bk);
         // two wavefronts per column: level 64 (the second one's first lane) takes level 63's values from LDS
         if (KW == 2 && k == KP - 1) {
             xch[3 * r] = tf[0];
@@ -543,7 +570,7 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             for (int f = 0; f < NFS; f++) nface[(f * (TJ + 1)) * KPW + k] = sf[f];
         }
         // top faces (s_top) of this level from the own column of the plane; w, wbar at k+1 by lane exchange
-        T W1 = __shfl(cc.w0, kp1), WB1 = __shfl(cc.wb0, kp1);
+        T W1 = lane_next(cc.w0), WB1 = lane_next(cc.wb0);
         if (KW == 2 && k == KP - 1) {
             W1 = xw[2 * r];
             WB1 = xw[2 * r + 1];
@@ -589,9 +616,9 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             }
             // rain top face with the fall speed (fluxes.cpp:734, 780; 966-1030)
             const T *QRr = SP(c0, 5) + RO * KPW;
-            const T v1l = __shfl(vtk, 1);
+            const T v1l = lane_read(vtk, 1);
             const T vte = k == 0 ? v1l : vtk;
-            T vk1 = __shfl(vte, kp1);
+            T vk1 = lane_next(vte);
             // two wavefronts per column: level 63 evaluates level 64's fall speed itself (NZ >= 66, so
             // level 64 is an interior level), from the same plane value the other wavefront uses
             if (KW == 2 && k == KP - 1) vk1 = fall(QRr[KP], KP);
@@ -612,8 +639,8 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         }
         T bot[NFS];
 #pragma unroll
-        for (int f = 0; f < NFS; f++) bot[f] = __shfl(tf[f], km1);
-        T qrb = MOIST ? __shfl(qrt, km1) : T(0);
+        for (int f = 0; f < NFS; f++) bot[f] = lane_prev(tf[f]);
+        T qrb = MOIST ? lane_prev(qrt) : T(0);
         if (KW == 2 && k == KP - 1) {
 #pragma unroll
             for (int f = 0; f < NFS; f++) xb[(NFS + 1) * r + f] = tf[f];
@@ -669,15 +696,16 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         // touches it); the boundary epilogue redoes the columns it rewrites whole.  The sources of the
         // periodic copy are left to it: halo column 0 receives column NX-6's values from before the lid.
         {
-            const T t1 = __shfl(oth, 1), tz = __shfl(oth, NZ - 2);
+            const int lz = (NZ - 2) & (KP - 1);  // lane of level NZ-2 in its wavefront
+            const T t1 = lane_read(oth, 1), tz = lane_read(oth, lz);
             T v1 = T(0), vz = T(0), c1 = T(0), cz = T(0), r1 = T(0), rz = T(0);
             if (MOIST) {
-                v1 = __shfl(oqv, 1);
-                vz = __shfl(oqv, NZ - 2);
-                c1 = __shfl(oqc, 1);
-                cz = __shfl(oqc, NZ - 2);
-                r1 = __shfl(oqr, 1);
-                rz = __shfl(oqr, NZ - 2);
+                v1 = lane_read(oqv, 1);
+                vz = lane_read(oqv, lz);
+                c1 = lane_read(oqc, 1);
+                cz = lane_read(oqc, lz);
+                r1 = lane_read(oqr, 1);
+                rz = lane_read(oqr, lz);
             }
             if (j < a.jh && i >= 6 && i < g.NX - 6 && (k == 0 || k == NZ - 1)) {
                 const unsigned pl = (unsigned)i * NYNZ + (unsigned)(jc * NZ + k);
@@ -881,7 +909,8 @@ __device__ __forceinline__ void post_uvw_col(const Geo &g, const Cols &c, const 
         w -= coef * w;
     }
     // lid from lanes 1 and NZ-2 (j >= 1; every source column has i >= 1)
-    T u1 = __shfl(u, 1), uz = __shfl(u, NZ - 2), v1 = __shfl(v, 1), vz = __shfl(v, NZ - 2);
+    const int lz = (NZ - 2) & (KP - 1);  // lane of level NZ-2 in its wavefront
+    T u1 = lane_read(u, 1), uz = lane_read(u, lz), v1 = lane_read(v, 1), vz = lane_read(v, lz);
     if (k >= NZ) return;
     T uf = u, vf = v, wf = w;
     if (j >= 1) {
@@ -928,7 +957,7 @@ __device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, co
         if (ray) {
             T *A = (T *)a.f[3];
             const T x = A[col + kc];
-            const T x1 = __shfl(x, 1), xz = __shfl(x, NZ - 2);
+            const T x1 = lane_read(x, 1), xz = lane_read(x, (NZ - 2) & (KP - 1));
             const bool lid = lid_interior && (k == 0 || k == NZ - 1);
             if (k < NZ && (lid || k >= a.ray_level)) {
                 T fin = k == 0 && lid ? x1 : (k == NZ - 1 && lid ? xz : x);
@@ -949,7 +978,7 @@ __device__ __forceinline__ void post_scalars_col(const Geo &g, const Cols &c, co
             if (x + qbp + T(c.qb[kc]) < T(0)) x = -(qbp + T(c.qb[kc]));
         }
         if (coef >= T(0) && kc >= 1) x -= coef * x;
-        T x1 = __shfl(x, 1), xz = __shfl(x, NZ - 2);
+        T x1 = lane_read(x, 1), xz = lane_read(x, (NZ - 2) & (KP - 1));
         if (k < NZ) {
             T fin = x;
             if (j >= 1) {
