@@ -216,6 +216,28 @@ def test_c2_full_size_properties():
     assert np.abs(inner).max() <= 1e-9 * scale
 
 
+@pytest.mark.parametrize("name,steps", [("supercell", 3), ("tropical_vortex", 2)])
+def test_full_size_fp64_against_oracle(name, steps):
+    """The FP64 form at the full moist sizes, 256x256x64 (supercell) and 512x512x80 (tropical vortex, two
+    wavefronts per column and the three-kernel j pass), against the OpenMP build of the oracle: every role
+    of every prognostic, pi and the rain total to 1e-11 after a few steps (the oracle's cost bounds the
+    step count)."""
+    c = configs.supercell() if name == "supercell" else configs.tropical_vortex()
+    print(f"full-size {name} {c.nx}x{c.ny}x{c.nz}, {steps} steps", flush=True)
+    g = DynCore(c)
+    g.step(steps)
+    g.sync()
+    o = Oracle(c, variant="omp")
+    o.step(steps)
+    for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR):
+        for role in (ids.ROLE_M, CUR, P):
+            e = rel_err(g.get_field(f, role), o.get_field(f, role))
+            assert e <= 1e-11, (ids.FIELD_NAMES[f], role, e)
+    assert rel_err(g.get_field(ids.PI, 0), o.get_field(ids.PI, 0)) <= 1e-11
+    assert rel_err(g.get_field(ids.ACCRAIN), o.get_field(ids.ACCRAIN)) <= 1e-11
+    g.close()
+
+
 def test_c2_fp32_one_hour_error_against_fp64():
     """The FP32 form against the FP64 form (which the oracle pins) over the full 1800 steps (1 h) of the
     256x256x64 moist supercell: the relative L2 error of every prognostic and of pi, reported every 300
