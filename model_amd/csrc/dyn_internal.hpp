@@ -71,6 +71,9 @@ struct PressurePlan {
     void *spec = nullptr;              // spectral work array [kx][ky][kk] complex (T)
     double *ffac = nullptr;            // Thomas multipliers F2 [kx][kk][ky] (FP64)
     double *zpiv = nullptr;            // Thomas pivot reciprocals [kx][kk][ky] (FP64)
+    // transform lengths whose factors are all 2, 3 and 5 (the in-place LDS transforms' butterflies); a
+    // length with a larger prime factor takes the out-of-place transforms, whose passes sum any radix
+    bool smooth_x = true, smooth_y = true;
     bool fused_y = false;              // j-transforms and the vertical solve in one LDS-resident kernel
     bool half_y = false;               // ... or in two per kx on half slabs (even nj; one radix-2 split)
     void *spec2 = nullptr;             // output of the half-slab j pass ([kx][j][kk] complex, T)

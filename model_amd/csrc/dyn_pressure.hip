@@ -127,8 +127,35 @@ __device__ __forceinline__ void butterfly8(T *xr, T *xi, int sign) {
     xi[7] = ei[3] - t3i;
 }
 
+// Output q of butterfly jj of a Stockham pass of prime radix R > 5, summed directly: the butterfly's
+// inputs x_r (element jj + r n/R), each twiddled by W^(r jm stride), times omega_R^(r q) = W^(r q n/R),
+// i.e. sum_r x_r W^(r E mod n) with E = jm stride + q n/R.  The R-term sum runs in FP64 whatever the
+// storage type (its rounding grows with R, where a factored transform's grows with log R).  `at(e)` is
+// the twiddle W^e as (cos, -sin) of 2 pi e / n; `in(m)` returns element m of the line.
+template <typename In, typename Tw>
+__device__ __forceinline__ void dft_radix(int n, int R, int jj, int q, int Ns, int stride, int sign, In in, Tw at, double &yr,
+                                          double &yi) {
+    const int nbf = n / R, jm = jj % Ns;
+    const int E = (int)(((long long)jm * stride + (long long)q * nbf) % n);
+    double ar = 0.0, ai = 0.0;
+    int e = 0;
+    for (int r = 0; r < R; r++) {
+        double xr, xi, wr, wi;
+        in(jj + r * nbf, xr, xi);
+        at(e, wr, wi);
+        if (sign > 0) wi = -wi;
+        ar += xr * wr - xi * wi;
+        ai += xr * wi + xi * wr;
+        e += E;
+        if (e >= n) e -= n;
+    }
+    yr = ar;
+    yi = ai;
+}
+
 // Stockham DFT of kc interleaved batches held in LDS as [n][kc] (re and im planes); ping-pongs
-// between buffer 0 and 1 and returns the index of the buffer holding the result.
+// between buffer 0 and 1 and returns the index of the buffer holding the result.  Radices 2 to 5 take
+// the butterflies; a prime radix above 5 takes the direct sums of dft_radix, one output per work item.
 template <typename T>
 __device__ int lds_fft(T *re[2], T *im[2], const Radix &rad, int kc, const double *tw, int sign) {
     const int n = rad.n;
@@ -138,6 +165,22 @@ __device__ int lds_fft(T *re[2], T *im[2], const Radix &rad, int kc, const doubl
         const int nb = n / R;
         const int stride = n / (Ns * R);
         T *ir = re[cur], *ii = im[cur], *orr = re[cur ^ 1], *oi = im[cur ^ 1];
+        if (R > 5) {
+            for (int idx = threadIdx.x; idx < n * kc; idx += blockDim.x) {
+                const int b = idx % kc, o = idx / kc, jj = o % nb, q = o / nb;
+                double yr, yi;
+                dft_radix(
+                    n, R, jj, q, Ns, stride, sign, [&](int m, double &xr, double &xi) { xr = double(ir[m * kc + b]); xi = double(ii[m * kc + b]); },
+                    [&](int e, double &wr, double &wi) { wr = tw[2 * e]; wi = tw[2 * e + 1]; }, yr, yi);
+                const int od = (jj / Ns) * Ns * R + jj % Ns;
+                orr[(od + q * Ns) * kc + b] = T(yr);
+                oi[(od + q * Ns) * kc + b] = T(yi);
+            }
+            __syncthreads();
+            cur ^= 1;
+            Ns *= R;
+            continue;
+        }
         for (int idx = threadIdx.x; idx < nb * kc; idx += blockDim.x) {
             int b = idx % kc, jj = idx / kc;
             T xr[5], xi[5];
@@ -1260,13 +1303,18 @@ __global__ void __launch_bounds__(128) k_thomas_staged(T *__restrict__ spec, int
     }
 }
 
+// Radix passes of a length-n transform: 4, 2, 3 and 5 first (the butterflies), then the remaining prime
+// factors in increasing order (direct sums, dft_radix); false when n needs more than kMaxRad passes.
 bool factorize(int n, Radix &rad) {
     rad.n = n;
     rad.nr = 0;
     int m = n;
     while (m > 1 && rad.nr < kMaxRad) {
         int r = (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : 0;
-        if (r == 0) return false;
+        if (r == 0) {
+            r = 7;
+            while (m % r != 0) r++;  // smallest prime factor of m (every smaller factor was divided out)
+        }
         rad.r[rad.nr++] = r;
         m /= r;
     }
@@ -1333,9 +1381,14 @@ double *upload_twiddles(int n, hipStream_t s) {
     return d;
 }
 
+// butterflies a thread holds in the worst pass; a pass of prime radix above 5 holds outputs instead, 5
+// per radix-5 butterfly's worth of registers
 int max_butterflies(const Radix &r, int nbatch, int nthr) {
     int mb = 0;
-    for (int s = 0; s < r.nr; s++) mb = std::max(mb, (r.n / r.r[s] * nbatch + nthr - 1) / nthr);
+    for (int s = 0; s < r.nr; s++) {
+        if (r.r[s] > 5) mb = std::max(mb, ((r.n * nbatch + nthr - 1) / nthr + 4) / 5);
+        else mb = std::max(mb, (r.n / r.r[s] * nbatch + nthr - 1) / nthr);
+    }
     return mb;
 }
 
@@ -1372,7 +1425,8 @@ int solve_t(Ctx *c, double step, bool halos, PressureHook after_i, void *arg) {
     const double pts = (double)pp.ni * pp.nj * pp.nk2;
     ThomasArgs ta{pp.coskx, pp.cosly, c->cols.rhou, c->cols.rhow, c->cols.mu, c->cols.mw, pp.acoef, pp.ffac, c->g.dx, c->g.dz,
                   pp.ni, pp.nj, c->g.nz, pp.zpiv};
-    if (pp.ni % 2 == 0) {
+    // even ni of factors 2, 3, 5: half-spectrum transforms in place; otherwise the full complex spectrum
+    if (pp.ni % 2 == 0 && pp.smooth_x) {
         const int nkx = pp.ni / 2 + 1;
         const int kc = x_level_chunk(pp, g, rx, sizeof(T));
         const size_t shx = x_pass_lds(pp.ni, kc, sizeof(T));
@@ -1430,7 +1484,7 @@ int solve_t(Ctx *c, double step, bool halos, PressureHook after_i, void *arg) {
         DYN_LAUNCH_CHECK();
         return DYN_OK;
     }
-    // odd ni: full complex spectrum
+    // odd ni, or a prime factor above 5: full complex spectrum, out-of-place transforms
     int kcx = level_chunk(pp.ni, sizeof(T), pp.nk2), kcy = level_chunk(pp.nj, sizeof(T), pp.nk2);
     size_t shx = 4 * (size_t)pp.ni * kcx * sizeof(T), shy = 4 * (size_t)pp.nj * kcy * sizeof(T);
     dim3 gx(g.NY, (pp.nk2 + kcx - 1) / kcx), gy(pp.ni, (pp.nk2 + kcy - 1) / kcy);
@@ -1537,8 +1591,9 @@ template <typename T>
 int solve_adjoint_t(Ctx *c, double step, T *pihat, T *uh, T *vh, T *wh, T *r1, T *r2) {
     PressurePlan &pp = c->pp;
     const Geo &g = c->geo;
-    if (pp.ni % 2 != 0) {
-        set_error("adjoint pressure solve: needs an even ni (half-spectrum transform)");
+    if (pp.ni % 2 != 0 || !pp.smooth_x) {
+        set_error("adjoint pressure solve: needs an even ni of factors 2, 3 and 5 (half-spectrum transform), got ni=" +
+                  std::to_string(pp.ni));
         return DYN_E_UNSUPPORTED;
     }
     Radix rx{}, ry{};
@@ -1617,10 +1672,17 @@ int pressure_init(Ctx *c) {
     pp.nk2 = NZ - 2;
     Radix rx{}, ry{};
     if (!factorize(pp.ni, rx) || !factorize(pp.nj, ry)) {
-        set_error("pressure solver: transform sizes must factor into 2, 3, 4 and 5 (ni=" + std::to_string(pp.ni) +
+        set_error("pressure solver: transform sizes need at most 8 radix passes (ni=" + std::to_string(pp.ni) +
                   ", nj=" + std::to_string(pp.nj) + ")");
         return DYN_E_UNSUPPORTED;
     }
+    auto smooth = [](const Radix &r) {
+        for (int q = 0; q < r.nr; q++)
+            if (r.r[q] > 5) return false;
+        return true;
+    };
+    pp.smooth_x = smooth(rx);
+    pp.smooth_y = smooth(ry);
     pp.twx = upload_twiddles(pp.ni, c->stream);
     pp.twy = upload_twiddles(pp.nj, c->stream);
     if (!pp.twx || !pp.twy) {
@@ -1639,7 +1701,7 @@ int pressure_init(Ctx *c) {
     const bool f32 = c->prec == DYN_FP32;
     const int nthr = f32 ? ypass_threads<float>() : ypass_threads<double>();
     const int maxb = f32 ? ypass_maxb<float>() : ypass_maxb<double>();
-    pp.fused_y = need <= pp.lds_max && 2 * pp.nj <= nthr && max_butterflies(ry, pp.nk2, nthr) <= maxb;
+    pp.fused_y = pp.smooth_y && need <= pp.lds_max && 2 * pp.nj <= nthr && max_butterflies(ry, pp.nk2, nthr) <= maxb;
     if (pp.fused_y && f32) {
         DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
         DYN_HIP(hipFuncSetAttribute((const void *)k_ypass<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
@@ -1660,7 +1722,7 @@ int pressure_init(Ctx *c) {
             const int items = rh.n / rh.r[st] * pp.nk2, per = (items + hthr - 1) / hthr;
             fits = per <= (rh.r[st] == 8 ? pow2_maxb8(hmaxb) : hmaxb);
         }
-        pp.half_y = fits;
+        pp.half_y = fits && pp.smooth_x;  // its inverse i pass (the row pairs) is an in-place transform
         if (pp.half_y) {
             DYN_HIP(hipMalloc(&pp.spec2, sizeof(double) * 2 * (size_t)pp.ni * pp.nj * pp.nk2));
             if (f32) {
@@ -1733,7 +1795,7 @@ int launch_solve_pressure(Ctx *c, double step, bool halos, PressureHook after_i,
 // solve (odd ni, or a (j, k) slab too large for the LDS-resident j pass)
 bool pressure_ens_fill(Ctx *c, EnsMember &e, size_t &lds_x, size_t &lds_y) {
     PressurePlan &pp = c->pp;
-    if (pp.ni % 2 != 0 || !pp.fused_y || c->prec != DYN_FP32) return false;
+    if (pp.ni % 2 != 0 || !pp.smooth_x || !pp.fused_y || c->prec != DYN_FP32) return false;
     if (!factorize(pp.ni, e.rx) || !factorize(pp.nj, e.ry)) return false;
     e.kcx = x_level_chunk(pp, c->geo, e.rx, sizeof(float), false);
     e.ni = pp.ni;

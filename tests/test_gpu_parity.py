@@ -216,6 +216,54 @@ def test_c2_full_size_properties():
     assert np.abs(inner).max() <= 1e-9 * scale
 
 
+# transform lengths with prime factors above 5, as the reference's own inputs have them (ni = NX - 6):
+# 14 = 2*7, 19, 46 = 2*23, 194 = 2*97 along i; 22 = 2*11, 26 = 2*13, 38 = 2*19, 100 along j
+PRIME_GRIDS = [(14, 22, 12, 1), (19, 26, 12, 0), (46, 38, 16, 1), (194, 100, 20, 1)]
+
+
+@pytest.mark.parametrize("nx,ny,nz,moist", PRIME_GRIDS, ids=lambda v: str(v))
+def test_prime_transform_lengths_against_oracle(nx, ny, nz, moist):
+    """Grids whose transform lengths have prime factors the butterflies do not cover: the solve takes the
+    out-of-place transforms, whose passes sum any prime radix directly; the pressure operator to 1e-13
+    and three whole steps to 1e-11 in every role."""
+    c = configs.small_case(nx=nx, ny=ny, nz=nz, microphysics=moist, amplitude=0.2, rayleigh_level=nz - 4)
+    print(f"prime lengths ni={nx} nj={ny} nz={nz} moist={moist}", flush=True)
+    o, g = Oracle(c), DynCore(c)
+    o.solve_pressure(0.5)
+    g.solve_pressure(0.5)
+    g.sync()
+    for f in (ids.PI, ids.U, ids.V, ids.W):
+        role = 0 if f == ids.PI else P
+        e = rel_err(g.get_field(f, role), o.get_field(f, role))
+        assert e <= 1e-13, (ids.FIELD_NAMES[f], e)
+    g.close()
+    o, g = Oracle(c), DynCore(c)
+    o.step(3)
+    g.step(3)
+    g.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH) + ((ids.QV, ids.QC, ids.QR) if moist else ()):
+        for role in (ids.ROLE_M, CUR, P):
+            e = rel_err(g.get_field(f, role), o.get_field(f, role))
+            assert e <= 1e-11, (ids.FIELD_NAMES[f], role, e)
+    assert rel_err(g.get_field(ids.PI, 0), o.get_field(ids.PI, 0)) <= 1e-11
+    g.close()
+
+
+def test_prime_transform_lengths_fp32():
+    """The FP32 form at 2*97 x 100 against the FP64 oracle after three steps: the direct sums run in FP64,
+    so the FP32 error stays at storage rounding."""
+    c = configs.small_case(nx=194, ny=100, nz=20, microphysics=1, amplitude=0.2, rayleigh_level=16)
+    o, g = Oracle(c), DynCore(c, precision=DYN_FP32)
+    o.step(3)
+    g.step(3)
+    g.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH):
+        e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
+        print(f"fp32 194x100x20 {ids.FIELD_NAMES[f]}: {e:.2e}", flush=True)
+        assert e <= 1e-5, (ids.FIELD_NAMES[f], e)
+    g.close()
+
+
 @pytest.mark.parametrize("name,steps", [("supercell", 3), ("tropical_vortex", 2)])
 def test_full_size_fp64_against_oracle(name, steps):
     """The FP64 form at the full moist sizes, 256x256x64 (supercell) and 512x512x80 (tropical vortex, two

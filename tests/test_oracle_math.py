@@ -12,7 +12,8 @@ from model_amd import configs, ids
 from oracle.pyoracle import Oracle, _ptr
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 12, 14, 60, 64, 96, 128, 160, 192, 256, 512])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 12, 14, 19, 22, 26, 38, 46, 60, 64, 94, 96, 128, 160, 184, 190, 192, 194,
+                               256, 512, 777, 1029])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft1d_matches_numpy(oracle_lib, n, sign):
     rng = np.random.default_rng(n)
@@ -25,7 +26,7 @@ def test_fft1d_matches_numpy(oracle_lib, n, sign):
     assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (96, 128), (160, 192), (256, 256)])
+@pytest.mark.parametrize("shape", [(64, 64), (96, 128), (160, 192), (256, 256), (194, 100), (184, 190)])
 def test_fft2d_matches_numpy(oracle_lib, shape):
     n0, n1 = shape
     rng = np.random.default_rng(7)
