@@ -164,9 +164,11 @@ void *dyn_stream(dyn_ctx *ctx);
  * GPU, never what is computed).  0 restores the built-in default; a value the kernels do not support
  * returns DYN_E_ARG.  Setting a knob drops the captured step graph. */
 enum {
-    DYN_TUNE_MOMENTUM_TJ = 0, /* j-rows per block of the momentum stage kernel: 2, 4 or 8 */
+    DYN_TUNE_MOMENTUM_TJ = 0, /* j-rows per block of the momentum stage kernel: 2, 4, 8 or 16 (16, like 8,
+                                 only with one wavefront per column) */
     DYN_TUNE_MOMENTUM_TI = 1, /* columns one momentum block marches along i: 2 .. 512 */
-    DYN_TUNE_SCALAR_TJ = 2,   /* j-rows per block of the scalar (theta, moisture) stage kernel: 2, 4 or 8 */
+    DYN_TUNE_SCALAR_TJ = 2,   /* j-rows per block of the scalar (theta, moisture) stage kernel: 2, 4, 8 or 12
+                                 (8 and 12 only with one wavefront per column) */
     DYN_TUNE_SCALAR_TI = 3,   /* columns one scalar block marches along i: 2 .. 512 */
     DYN_TUNE_PRESSURE_JPASS = 4, /* pressure j pass: 1 one LDS-resident kernel per kx (when the slab fits), 2 three
                                     kernels, 3 two LDS-resident kernels per kx on half slabs (even NY, when they fit) */

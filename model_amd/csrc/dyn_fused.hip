@@ -195,8 +195,8 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
             }
         }
     };
-    // own-column inputs of the buoyancy term (theta', qv, qc, qr at column i)
-    T cb[4] = {T(0), T(0), T(0), T(0)}, sb[4] = {T(0), T(0), T(0), T(0)};
+    // own-column inputs of column i: the buoyancy term's theta', qv, qc, qr and the M values of u, v, w
+    T cb[7] = {}, sb[7] = {};
     auto fetch_col = [&](int i, T *b) {
         const unsigned p = (unsigned)i * NYNZ + rowoff;
         b[0] = ld<T>(a.cur[3], p);
@@ -205,6 +205,9 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
             b[2] = ld<T>(a.cur[5], p);
             b[3] = ld<T>(a.cur[6], p);
         }
+        b[4] = ld<T>(a.mprev[0], p);
+        b[5] = ld<T>(a.mprev[1], p);
+        b[6] = ld<T>(a.mprev[2], p);
     };
 
     // prologue: queues at columns i0-3 .. i0+2, planes U(i0, i0+1), V/W(i0-1, i0)
@@ -222,6 +225,8 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
     const T DTZ = T(g.dtz) * T(c.mu[kc]);
     const T rw1 = T(c.rhow[kp1]), rw0 = T(c.rhow[kc]), odr = T(c.one_d_rhou[kc]);
     const T fj = T(c.f[jc]);
+    // read once: a load inside the march would be waited for together with the next column's prefetch
+    const T tbvk = T(c.tbv[kc]);
     const bool low = (k == 1 || k == NZ - 2);
 
     // east faces of the queue's centre column (u_east, v_east; fluxes.cpp:1384-1592); Ujm1/UBjm1: row j-1 of
@@ -251,19 +256,22 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
         for (int d = 0; d < 5; d++) q[f][d] = q[f][d + 1];
         q[f][5] = ld<T>(qsrc[f], (unsigned)(i0 + 3) * NYNZ + rowoff);
     }
+    // Every prologue load has landed before the march starts: a load still outstanding at the loop's
+    // entry makes the compiler wait, inside every iteration, for the next column's prefetch as well.
+    __builtin_amdgcn_s_waitcnt(0);
 
     for (int i = i0; i < i1; i++) {
         const bool more = i + 1 < i1;
         const int c0 = i % 3, cm1 = (i + 2) % 3, cp1 = (i + 1) % 3;
         const unsigned p = (unsigned)i * NYNZ + rowoff;
-        // loads for the next column go out first; this column's M values next
+        // loads for the next column go out first, its M values included
         if (more) {
 #pragma unroll
             for (int f = 0; f < 4; f++) qs[f] = ld<T>(qsrc[f], (unsigned)(i + 4) * NYNZ + rowoff);
             fetch_planes(i + 2, i + 1);
             fetch_col(i + 1, sb);
         }
-        const T um = ld<T>(a.mprev[0], p), vm = ld<T>(a.mprev[1], p), wm = ld<T>(a.mprev[2], p);
+        const T um = cb[4], vm = cb[5], wm = cb[6];
 
         east_faces(UV(cp1, 0, RO - 1), UV(cp1, 1, RO - 1), east);
         // north faces (u_north, v_north) of row RR of the U/V planes
@@ -332,8 +340,8 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
         if (k == 0) tf[0] = tf[1] = T(0);
         // bottom face of level k = top face of level k-1; buoyancy of level k-1 (advection.cpp:42-58)
         T bu = lane_prev(tf[0]), bv = lane_prev(tf[1]);
-        T bk = cb[0] / T(c.tbv[kc]);
-        if (MOIST) bk = cb[0] / T(c.tbv[kc]) + T(0.61) * cb[1] - cb[2] - cb[3];
+        T bk = cb[0] / tbvk;
+        if (MOIST) bk = cb[0] / tbvk + T(0.61) * cb[1] - cb[2] - cb[3];
         T bkm = lane_prev(bk);
         // two wavefronts per column: level 64 (the second one's first lane) takes level 63's values from LDS
         if (KW == 2 && k == KP - 1) {
@@ -347,21 +355,22 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
             bv = xch[3 * r + 1];
             bkm = xch[3 * r + 2];
         }
+        T ou = T(0), ov = T(0), ow = T(0);  // this level's new values, stored after the barrier below
         if (active) {
             const T su0 = nface[(0 * (TJ + 1) + r) * KPW + k], sv0 = nface[(1 * (TJ + 1) + r) * KPW + k];
             {  // u (advection.cpp:73-99)
                 const T *V0 = UV(c0, 2, 0), *Vm = UV(cm1, 2, 0);
                 T other = T(0.25) * (V0[RO * KPW + k] + V0[(RO + 1) * KPW + k] + Vm[RO * KPW + k] + Vm[(RO + 1) * KPW + k]);
                 T tend = -dtx * (east[0] - west[0]) - dty * (nf[0] - su0) - DTZ * (rw1 * tf[0] - rw0 * bu) * odr + dt * fj * other;
-                st<T>(a.out[0], p, um + step * tend);
+                ou = um + step * tend;
             }
             {  // v (advection.cpp:104-128)
                 const T *U0 = UV(c0, 0, 0), *Up = UV(cp1, 0, 0);
                 T other = T(0.25) * (U0[RO * KPW + k] + Up[RO * KPW + k] + U0[(RO - 1) * KPW + k] + Up[(RO - 1) * KPW + k]);
                 T tend = -dtx * (east[1] - west[1]) - dty * (nf[1] - sv0) - DTZ * (rw1 * tf[1] - rw0 * bv) * odr - dt * fj * other;
-                st<T>(a.out[1], p, vm + step * tend);
+                ov = vm + step * tend;
             }
-            if (k >= 2) st<T>(a.out[2], p, wm + step * dt * T(kGrav) * T(0.5) * (bk + bkm));  // advection.cpp:329-337
+            ow = wm + step * dt * T(kGrav) * T(0.5) * (bk + bkm);  // advection.cpp:329-337
         }
         west[0] = east[0];
         west[1] = east[1];
@@ -374,9 +383,16 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
                 q[f][5] = qs[f];
             }
 #pragma unroll
-            for (int f = 0; f < 4; f++) cb[f] = sb[f];
+            for (int f = 0; f < 7; f++) cb[f] = sb[f];
         }
         __syncthreads();
+        // stored after the next column's planes are written: a store issued earlier would hold up the
+        // wait for the next column's prefetched values (the memory counter completes them in order)
+        if (active) {
+            st<T>(a.out[0], p, ou);
+            st<T>(a.out[1], p, ov);
+            if (k >= 2) st<T>(a.out[2], p, ow);
+        }
     }
 }
 
@@ -456,9 +472,16 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             }
         }
     };
-    // own-column inputs of column i: u, ubar at i+1; v, vbar at j+1 and j; w, wbar; VT at the top level
+    // Register budget: in FP32 the M values of theta and q and qbar travel with the column's prefetch, and
+    // a column's results are stored after the end-of-column barrier (so no store is counted ahead of the
+    // next column's loads).  In FP64 both would push the kernel further into scratch spills, whose
+    // reloads wait on the memory counter too: there the M values are loaded when the column starts and
+    // the results stored before the plane update (1.90 against 1.99 ms per moist 256x256x64 step).
+    constexpr bool kLean = sizeof(T) == 8;
+    // own-column inputs of column i: u, ubar at i+1; v, vbar at j+1 and j; w, wbar; VT at the top level;
+    // the M values of theta and q and qbar (FP32)
     struct Col {
-        T u1, ub1, vn, vbn, vs, vbs, w0, wb0, vtt;
+        T u1, ub1, vn, vbn, vs, vbs, w0, wb0, vtt, mth, mqv, mqc, mqr, qbp;
     };
     auto fetch_col = [&](int i, Col &o) {
         const unsigned p = (unsigned)i * NYNZ + rowoff, pe = (unsigned)(i + 1) * NYNZ + rowoff, pn = (unsigned)i * NYNZ + rowoffn;
@@ -473,6 +496,15 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
         o.w0 = ld<T>(a.cur[2], p);
         o.wb0 = ld<T>(a.bar[2], p);
         if (MOIST) o.vtt = ld<T>(a.vt, (unsigned)i * NYNZ + topoff);
+        if (!kLean) {
+            o.mth = ld<T>(a.mprev[3], p);
+            if (MOIST) {
+                o.mqv = ld<T>(a.mprev[4], p);
+                o.mqc = ld<T>(a.mprev[5], p);
+                o.mqr = ld<T>(a.mprev[6], p);
+                o.qbp = ld<T>(a.bar[4], p);
+            }
+        }
     };
 
 #pragma unroll
@@ -484,6 +516,23 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
     const T DTZ = T(g.dtz) * T(c.mu[kc]);
     const T rw1 = T(c.rhow[kp1]), rw0 = T(c.rhow[kc]), odr = T(c.one_d_rhou[kc]);
     const bool low = (k == 1 || k == NZ - 2);
+    // The remaining column constants of this lane's level are read here, once.  A load inside the march
+    // is waited for together with every load issued before it (the memory counter completes in order),
+    // so one there would also wait for the next column's prefetch.  Indices are clamped for the lanes
+    // outside the active levels, which never use the values.
+    const T tbu = T(c.tb[kp1]) - T(c.tb[kc]), tbd = T(c.tb[kc]) - T(c.tb[km1]);
+    T qbu = T(0), qbd = T(0), qbk = T(0), rk = T(1), vtden = T(0), rk64 = T(1), vtden64 = T(0);
+    if (MOIST) {
+        qbu = T(c.qb[kp1]) - T(c.qb[kc]);
+        qbd = T(c.qb[kc]) - T(c.qb[km1]);
+        qbk = T(c.qb[kc]);
+        rk = T(c.rhou[kc]);
+        vtden = sqrt(T(c.rhou[1]) / rk);
+        if (KW == 2) {  // level 64, whose fall speed level 63 evaluates
+            rk64 = T(c.rhou[KP]);
+            vtden64 = sqrt(T(c.rhou[1]) / rk64);
+        }
+    }
 
     // east faces (s_east; fluxes.cpp:797-1032) of the queue's centre column with u, ubar at the column east of it
     auto east_faces = [&](T U1, T UB1, T *e) {
@@ -523,6 +572,11 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
     fetch_plane(i0);
     store_plane(i0, 2);
     __syncthreads();
+    // Every prologue load has landed before the march starts.  Without this wait the waves that skip the
+    // halo loads enter the loop with the own-column loads still counted as outstanding, and the compiler
+    // then waits inside every iteration (at the first use of those registers) for most of the next
+    // column's prefetch as well.
+    __builtin_amdgcn_s_waitcnt(0);
 
     for (int i = i0; i < i1; i++) {
         const bool more = i + 1 < i1;
@@ -534,12 +588,15 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             fetch_col(i + 1, cs);
             fetch_plane(i + 1);
         }
-        T mth = ld<T>(a.mprev[3], p), mqv = T(0), mqc = T(0), mqr = T(0), qbp = T(0);
-        if (MOIST) {
-            mqv = ld<T>(a.mprev[4], p);
-            mqc = ld<T>(a.mprev[5], p);
-            mqr = ld<T>(a.mprev[6], p);
-            qbp = ld<T>(a.bar[4], p);
+        T mth = cc.mth, mqv = cc.mqv, mqc = cc.mqc, mqr = cc.mqr, qbp = cc.qbp;
+        if (kLean) {
+            mth = ld<T>(a.mprev[3], p);
+            if (MOIST) {
+                mqv = ld<T>(a.mprev[4], p);
+                mqc = ld<T>(a.mprev[5], p);
+                mqr = ld<T>(a.mprev[6], p);
+                qbp = ld<T>(a.bar[4], p);
+            }
         }
 
         east_faces(cc.u1, cc.ub1, east);
@@ -599,18 +656,16 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
                 tf[2] = wb * tq;
             }
             // fall speed of this level (kessler.cpp:222-252); level 0 stores level 1's, NZ-1 keeps the array's
-            auto fall = [&](T qr, int kk) -> T {
+            auto fall = [&](T qr, T rho, T den) -> T {
                 T v = T(0);
                 if (qr > T(1.0e-12)) {
-                    T rk = T(c.rhou[kk]);
-                    T qrr = fmax(qr * T(0.001) * rk, T(0));
-                    T vtden = sqrt(T(c.rhou[1]) / rk);
-                    v = T(36.34) * (pow(qrr, T(0.1364))) * vtden;
+                    T qrr = fmax(qr * T(0.001) * rho, T(0));
+                    v = T(36.34) * (pow(qrr, T(0.1364))) * den;
                 }
                 return v;
             };
             if (k >= 1 && k <= NZ - 2) {
-                vtk = fall(q[5][2], k);
+                vtk = fall(q[5][2], rk, vtden);
             } else if (k == NZ - 1) {
                 vtk = cc.vtt;
             }
@@ -621,7 +676,7 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             T vk1 = lane_next(vte);
             // two wavefronts per column: level 63 evaluates level 64's fall speed itself (NZ >= 66, so
             // level 64 is an interior level), from the same plane value the other wavefront uses
-            if (KW == 2 && k == KP - 1) vk1 = fall(QRr[KP], KP);
+            if (KW == 2 && k == KP - 1) vk1 = fall(QRr[KP], rk64, vtden64);
             if (k == 0) {
                 T t0 = T(0.5) * (QRr[0] + QRr[1]);
                 qrt = -T(0.5) * (v1l + v1l) * t0;
@@ -657,21 +712,17 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             T south[NFS];
 #pragma unroll
             for (int f = 0; f < NFS; f++) south[f] = nface[(f * (TJ + 1) + r) * KPW + k];
-            // Z_ADVECT_BASE (advection.cpp:16-18)
-            auto zbase = [&](const double *base) -> T {
-                return (T(0.50) * (T(c.rhow[k + 1]) * W1 * (T(base[k + 1]) - T(base[k])) +
-                                   T(c.rhow[k]) * cc.w0 * (T(base[k]) - T(base[k - 1]))) *
-                        T(c.one_d_rhou[k]));
-            };
+            // Z_ADVECT_BASE (advection.cpp:16-18) with the base-state differences du = b[k+1] - b[k],
+            // dd = b[k] - b[k-1] of this level
+            auto zbase = [&](T du, T dd) -> T { return (T(0.50) * (rw1 * W1 * du + rw0 * cc.w0 * dd) * odr); };
             {  // theta (advection.cpp:152-168)
                 T tend = -dtx * (east[0] - west[0]) - dty * (nf[0] - south[0]) - DTZ * (rw1 * tf[0] - rw0 * bot[0]) * odr -
-                         DTZ * zbase(c.tb);
+                         DTZ * zbase(tbu, tbd);
                 oth = mth + step * tend;
-                st<T>(a.out[3], p, oth);
             }
             if (MOIST) {  // advection.cpp:197-253
                 T tq = -dtx * (east[1] - west[1]) - dty * (nf[1] - south[1]) - DTZ * (rw1 * tf[1] - rw0 * bot[1]) * odr -
-                       DTZ * zbase(c.qb);
+                       DTZ * zbase(qbu, qbd);
                 T qv = mqv + step * tq;
                 T tc = -dtx * (east[2] - west[2]) - dty * (nf[2] - south[2]) - DTZ * (rw1 * tf[2] - rw0 * bot[2]) * odr;
                 T qc = mqc + step * tc;
@@ -680,23 +731,22 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
                 // zero_moisture for the advected points (microphysics.cpp:677-698)
                 if (qc < T(0)) qc = T(0);
                 if (qrv < T(0)) qrv = T(0);
-                if (qv + qbp + T(c.qb[k]) < T(0)) qv = -(qbp + T(c.qb[k]));
-                st<T>(a.out[4], p, qv);
-                st<T>(a.out[5], p, qc);
-                st<T>(a.out[6], p, qrv);
+                if (qv + qbp + qbk < T(0)) qv = -(qbp + qbk);
                 oqv = qv;
                 oqc = qc;
                 oqr = qrv;
-                st<T>(a.vt, p, vtk);
-                if (k == 1) st<T>(a.vt, p - 1, vtk);
             }
         }
         // the lid of upper_lower_boundaries (boundaries.cpp:304-312) for this column: level 0 takes level 1,
         // NZ-1 takes NZ-2.  These are the final values of an interior column (no sponge or periodic copy
         // touches it); the boundary epilogue redoes the columns it rewrites whole.  The sources of the
         // periodic copy are left to it: halo column 0 receives column NX-6's values from before the lid.
+        // A lane writes either its own level (active levels) or a lid level (levels 0 and NZ-1).
+        unsigned ps = p;
+        const bool lid = j < a.jh && i >= 6 && i < g.NX - 6 && (k == 0 || k == NZ - 1);
         {
             const int lz = (NZ - 2) & (KP - 1);  // lane of level NZ-2 in its wavefront
+            const int src = k == 0 ? 1 : lz;
             const T t1 = lane_read(oth, 1), tz = lane_read(oth, lz);
             T v1 = T(0), vz = T(0), c1 = T(0), cz = T(0), r1 = T(0), rz = T(0);
             if (MOIST) {
@@ -707,17 +757,30 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
                 r1 = lane_read(oqr, 1);
                 rz = lane_read(oqr, lz);
             }
-            if (j < a.jh && i >= 6 && i < g.NX - 6 && (k == 0 || k == NZ - 1)) {
-                const unsigned pl = (unsigned)i * NYNZ + (unsigned)(jc * NZ + k);
-                const bool bot_lvl = k == 0;
-                st<T>(a.out[3], pl, bot_lvl ? t1 : tz);
-                if (MOIST) {
-                    st<T>(a.out[4], pl, bot_lvl ? v1 : vz);
-                    st<T>(a.out[5], pl, bot_lvl ? c1 : cz);
-                    st<T>(a.out[6], pl, bot_lvl ? r1 : rz);
-                }
+            if (lid) {
+                const bool bot_lvl = src == 1;
+                ps = (unsigned)i * NYNZ + (unsigned)(jc * NZ + k);
+                oth = bot_lvl ? t1 : tz;
+                oqv = bot_lvl ? v1 : vz;
+                oqc = bot_lvl ? c1 : cz;
+                oqr = bot_lvl ? r1 : rz;
             }
         }
+        auto store_column = [&]() {
+            if (active || lid) {
+                st<T>(a.out[3], ps, oth);
+                if (MOIST) {
+                    st<T>(a.out[4], ps, oqv);
+                    st<T>(a.out[5], ps, oqc);
+                    st<T>(a.out[6], ps, oqr);
+                }
+            }
+            if (MOIST && active) {
+                st<T>(a.vt, p, vtk);
+                if (k == 1) st<T>(a.vt, p - 1, vtk);
+            }
+        };
+        if (kLean) store_column();
 #pragma unroll
         for (int f = 0; f < NFS; f++) west[f] = east[f];
         if (more) {
@@ -732,6 +795,10 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             publish_w();
         }
         __syncthreads();
+        // FP32: this column's stores go out after the next column's planes are written.  The memory
+        // counter completes loads and stores in order, so a store issued before that point would hold up
+        // the wait for the next column's prefetched values; here it completes during the next column.
+        if (!kLean) store_column();
     }
 }
 
@@ -823,11 +890,13 @@ int stage_t(Ctx *c, double step, int il, int ih, int jl, int jh, int parts) {
     if (parts & kStageMomentum) {
         if (mtj == 2) rc = momentum_t<T, 2, MOIST, 1>(c, a, T(step), mti);
         else if (mtj == 4) rc = momentum_t<T, 4, MOIST, 1>(c, a, T(step), mti);
+        else if (mtj == 16) rc = momentum_t<T, 16, MOIST, 1>(c, a, T(step), mti);
         else rc = momentum_t<T, 8, MOIST, 1>(c, a, T(step), mti);
     }
     if (rc || !(parts & kStageScalars)) return rc;
     if (stj == 2) return scalars_t<T, 2, MOIST, 1>(c, a, T(step), sti);
     if (stj == 4) return scalars_t<T, 4, MOIST, 1>(c, a, T(step), sti);
+    if (stj == 12) return scalars_t<T, 12, MOIST, 1>(c, a, T(step), sti);
     return scalars_t<T, 8, MOIST, 1>(c, a, T(step), sti);
 }
 
@@ -1185,6 +1254,8 @@ template <typename T, int KW>
 size_t stage_lds_t(bool momentum, bool moist, int tj) {
     if (tj == 2) return momentum ? MomLds<T, 2, KW>::bytes : moist ? ScalLds<T, 2, true, KW>::bytes : ScalLds<T, 2, false, KW>::bytes;
     if (tj == 4) return momentum ? MomLds<T, 4, KW>::bytes : moist ? ScalLds<T, 4, true, KW>::bytes : ScalLds<T, 4, false, KW>::bytes;
+    if (tj == 12) return moist ? ScalLds<T, 12, true, KW>::bytes : ScalLds<T, 12, false, KW>::bytes;  // scalar kernel only
+    if (tj == 16) return MomLds<T, 16, KW>::bytes;  // momentum kernel only
     return momentum ? MomLds<T, 8, KW>::bytes : moist ? ScalLds<T, 8, true, KW>::bytes : ScalLds<T, 8, false, KW>::bytes;
 }
 // LDS bytes per block of a stage kernel with tj rows
@@ -1196,10 +1267,12 @@ size_t stage_lds(const Ctx *c, bool momentum, int tj) {
 }  // namespace
 
 // launch-shape knobs of the stage kernels: rows per block (instantiated widths: 2, 4, 8 with one wavefront
-// per column, 2, 4 with two; within the LDS) and march length
+// per column, and 16 for the momentum kernel, 12 for the scalar kernel; 2, 4 with two; within the LDS)
+// and march length
 bool fused_shape_ok(const Ctx *c, int param, int value) {
     if (param == DYN_TUNE_MOMENTUM_TJ || param == DYN_TUNE_SCALAR_TJ) {
-        const bool inst = value == 2 || value == 4 || (value == 8 && fused_waves(c) != 2);
+        const bool one = fused_waves(c) != 2, mom = param == DYN_TUNE_MOMENTUM_TJ;
+        const bool inst = value == 2 || value == 4 || (one && (value == 8 || value == (mom ? 16 : 12)));
         return inst && stage_lds(c, param == DYN_TUNE_MOMENTUM_TJ, value) <= c->pp.lds_max;
     }
     if (param == DYN_TUNE_MOMENTUM_TI || param == DYN_TUNE_SCALAR_TI) return value >= 2 && value <= 512;
@@ -1271,7 +1344,17 @@ int set_lds_limits_all(const Ctx *c) {
     int rc = set_lds_limits<T, 2, 1>(c);
     if (!rc) rc = set_lds_limits<T, 4, 1>(c);
     if (!rc) rc = set_lds_limits<T, 8, 1>(c);
-    return rc;
+    if (rc) return rc;
+    const size_t mom = MomLds<T, 16, 1>::bytes, sm = ScalLds<T, 12, true, 1>::bytes, sd = ScalLds<T, 12, false, 1>::bytes;
+    if (mom <= c->pp.lds_max) {
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, 16, true, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mom));
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_momentum<T, 16, false, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mom));
+    }
+    if (sm <= c->pp.lds_max)
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, 12, true, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    if (sd <= c->pp.lds_max)
+        DYN_HIP(hipFuncSetAttribute((const void *)k_adv_scalars<T, 12, false, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sd));
+    return DYN_OK;
 }
 
 // the stage kernels may use more than 64 KiB of LDS: raise the per-launch limit once, outside any capture

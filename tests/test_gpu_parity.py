@@ -396,8 +396,9 @@ def test_fused_fp64_against_oracle_supercell():
 
 
 @pytest.mark.parametrize("precision,shape", [(DYN_FP32, (2, 8, 4, 32)), (DYN_FP32, (4, 32, 8, 5)), (DYN_FP32, (8, 5, 2, 16)),
-                                             (0, (2, 16, 8, 7)), (0, (8, 7, 4, 64))],
-                         ids=["fp32-a", "fp32-b", "fp32-c", "fp64-a", "fp64-b"])
+                                             (DYN_FP32, (16, 11, 12, 9)), (0, (2, 16, 8, 7)), (0, (8, 7, 4, 64)),
+                                             (0, (4, 5, 12, 24))],
+                         ids=["fp32-a", "fp32-b", "fp32-c", "fp32-d", "fp64-a", "fp64-b", "fp64-c"])
 def test_fused_launch_shapes_bitwise(precision, shape):
     """Rows per block and march length change how the stage maps onto the GPU, never its values."""
     from model_amd import core
@@ -470,7 +471,7 @@ def test_tall_columns_fused_matches_per_operator_bitwise(precision, moist, nz):
                          ids=["fp32-2rows", "fp32-4rows", "fp64-dry-4rows"])
 def test_tall_column_launch_shapes_bitwise(precision, shape):
     """Two wavefronts per column (NZ = 90): 2 or 4 rows per block and any march length give the same
-    bits; 8 rows per block is not offered."""
+    bits; 8, 12 and 16 rows per block are not offered."""
     from model_amd import core
     c = configs.warm_bubble(nx=24, ny=20, nz=90)
     a, b = DynCore(c, precision=precision), DynCore(c, precision=precision)
@@ -478,8 +479,9 @@ def test_tall_column_launch_shapes_bitwise(precision, shape):
     for knob, v in zip(knobs, shape):
         b.tune_set(knob, v)
         assert b.tune_get(knob) == v
-    with pytest.raises(DynError):
-        b.tune_set(core.DYN_TUNE_SCALAR_TJ, 8)
+    for knob, v in ((core.DYN_TUNE_SCALAR_TJ, 8), (core.DYN_TUNE_SCALAR_TJ, 12), (core.DYN_TUNE_MOMENTUM_TJ, 16)):
+        with pytest.raises(DynError):
+            b.tune_set(knob, v)
     a.step_fused(3)
     b.step_fused(3)
     a.sync()
