@@ -385,6 +385,10 @@ __device__ __forceinline__ void adv_momentum_tile(const Geo &g, const Cols &c, c
 #pragma unroll
             for (int f = 0; f < 7; f++) cb[f] = sb[f];
         }
+        // Drained on every path, so the compiler does not carry this column's prefetch as possibly
+        // outstanding into the next iteration (where it would wait on it, and on the stores below, at the
+        // first reuse of those registers).  On the main path the plane update has already waited.
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         // stored after the next column's planes are written: a store issued earlier would hold up the
         // wait for the next column's prefetched values (the memory counter completes them in order)
@@ -794,6 +798,8 @@ __device__ __forceinline__ void adv_scalars_tile(const Geo &g, const Cols &c, co
             cc = cs;
             publish_w();
         }
+        // drained on every path (see the momentum kernel)
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         // FP32: this column's stores go out after the next column's planes are written.  The memory
         // counter completes loads and stores in order, so a store issued before that point would hold up
