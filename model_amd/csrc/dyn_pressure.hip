@@ -11,6 +11,7 @@
 // chunk of kc consecutive levels so every global access is a contiguous kc-run of a k column.
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "dyn_internal.hpp"
@@ -707,33 +708,58 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
     load_twiddles(twl, tw, n);
     const int jj = j == 0 ? 1 : (j == g.NY - 1 ? g.NY - 2 : j);  // mirrored rows, pressure.cpp:1408-1415
     const double sdt = step * g.dt;
-    // element idx = i*kc + b; when kc divides the block size each thread keeps its level b
-    const bool fixed_b = (blockDim.x % kc) == 0;
-    const int di = blockDim.x / kc;
-    // unrolled: the global loads of several elements are in flight before their LDS stores
-#pragma unroll 4
-    for (int idx = threadIdx.x, b = threadIdx.x % kc, i = threadIdx.x / kc - di; idx < n * kc; idx += blockDim.x) {
-        i += di;
-        if (!fixed_b) {
-            b = idx % kc;
-            i = idx / kc;
+    // element idx = i*kc + b.  kU elements per thread per round: every global load of a round is issued
+    // (clamped levels, so no load sits behind a branch) before any of them is used; the source-field form
+    // and the divergence form are separate loops for the same reason.
+    constexpr int kU = 4;
+    const int ne = n * kc, nt = (int)blockDim.x;
+    auto gather = [&](auto from_src) {
+        constexpr bool kSrc = decltype(from_src)::value;
+        for (int idx0 = (int)threadIdx.x; idx0 < ne; idx0 += kU * nt) {
+            T f0[kU], f1[kU], f2[kU], f3[kU], f4[kU], f5[kU];
+            double ru[kU], rw1[kU], rw0[kU], mk[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int idx = min(idx0 + u * nt, ne - 1), b = idx % kc, i = idx / kc;
+                const int k = min(kk0 + b, nk2 - 1) + 1, ia = i + 3;
+                if (kSrc) {
+                    f0[u] = src[IX(ia, j, k)];
+                } else {
+                    const int ie = ia + 1 == g.NX - 3 ? 3 : ia + 1;
+                    f0[u] = UP[IX(ie, jj, k)];
+                    f1[u] = UP[IX(ia, jj, k)];
+                    f2[u] = VP[IX(ia, jj + 1, k)];
+                    f3[u] = VP[IX(ia, jj, k)];
+                    f4[u] = WP[IX(ia, jj, k + 1)];
+                    f5[u] = WP[IX(ia, jj, k)];
+                    ru[u] = c.rhou[k];
+                    rw1[u] = c.rhow[k + 1];
+                    rw0[u] = c.rhow[k];
+                    mk[u] = c.mu[k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int idx = idx0 + u * nt, b = idx % kc, i = idx / kc;
+                const int kk = kk0 + b;
+                T d = T(0);
+                if (kk < nk2 && kSrc) {
+                    d = f0[u];
+                } else if (kk < nk2) {
+                    T rhou = T(ru[u]);
+                    d = rhou * (f0[u] - f1[u]) * T(g.one_d_dx) + rhou * (f2[u] - f3[u]) * T(g.one_d_dy) +
+                        (T(rw1[u]) * f4[u] - T(rw0[u]) * f5[u]) * (T(g.one_d_dz) * T(mk[u]));
+                    d /= T(sdt);
+                }
+                if (idx < ne) {
+                    if (b & 1) im[i * P + (b >> 1)] = d;
+                    else re[i * P + (b >> 1)] = d;
+                }
+            }
         }
-        int kk = kk0 + b;
-        T d = T(0);
-        if (kk < nk2 && src) {
-            d = src[IX(i + 3, j, kk + 1)];
-        } else if (kk < nk2) {
-            int k = kk + 1, ia = i + 3;
-            int ie = ia + 1 == g.NX - 3 ? 3 : ia + 1;
-            T rhou = T(c.rhou[k]);
-            d = rhou * (UP[IX(ie, jj, k)] - UP[IX(ia, jj, k)]) * T(g.one_d_dx) +
-                rhou * (VP[IX(ia, jj + 1, k)] - VP[IX(ia, jj, k)]) * T(g.one_d_dy) +
-                (T(c.rhow[k + 1]) * WP[IX(ia, jj, k + 1)] - T(c.rhow[k]) * WP[IX(ia, jj, k)]) * (T(g.one_d_dz) * T(c.mu[k]));
-            d /= T(sdt);
-        }
-        if (b & 1) im[i * P + (b >> 1)] = d;
-        else re[i * P + (b >> 1)] = d;
-    }
+    };
+    if (src) gather(std::true_type{});
+    else gather(std::false_type{});
     __syncthreads();
     lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, twl, -1);
     // split the packed pair: A = (Z[m] + conj Z[-m]) / 2, B = (Z[m] - conj Z[-m]) / 2i
@@ -763,48 +789,74 @@ __device__ __forceinline__ void irfft_x_out_block(const Geo &g, const T *__restr
     T *re = (T *)smem, *im = re + n * P, *twl = im + n * P;
     const int kk0 = chunk * kc;
     load_twiddles(twl, tw, n);
-    // rebuild Z = A + iB for m in [0, n) from the half spectra of the two levels of each pair
-#pragma unroll 4
-    for (int idx = threadIdx.x; idx < n * P; idx += blockDim.x) {
-        int pp = idx % P, m = idx / P;
-        int ka = kk0 + 2 * pp, kb = ka + 1;
-        int ms = m <= half ? m : n - m;
-        long long o = ((long long)ms * g.NY + j) * nk2;
-        T ar = T(0), ai = T(0), br = T(0), bi = T(0);
-        if (jhalf) {
-            // the half-slab j pass left row j of the full inverse as v0(j') + s v1(j'), j' = j mod jhalf,
-            // with v0 in row j' and v1 in row jhalf + j' (s = +1 on the first half of the rows, -1 on the second)
-            const int jp = j < jhalf ? j : j - jhalf;
-            const T sg = j < jhalf ? T(1) : T(-1);
-            const long long o0 = ((long long)ms * g.NY + jp) * nk2, o1 = ((long long)ms * g.NY + jhalf + jp) * nk2;
-            if (ka < nk2) {
-                ar = spec[(o0 + ka) * 2] + sg * spec[(o1 + ka) * 2];
-                ai = spec[(o0 + ka) * 2 + 1] + sg * spec[(o1 + ka) * 2 + 1];
-            }
-            if (kb < nk2) {
-                br = spec[(o0 + kb) * 2] + sg * spec[(o1 + kb) * 2];
-                bi = spec[(o0 + kb) * 2 + 1] + sg * spec[(o1 + kb) * 2 + 1];
-            }
-        } else {
-            if (ka < nk2) {
-                ar = spec[(o + ka) * 2];
-                ai = spec[(o + ka) * 2 + 1];
-            }
-            if (kb < nk2) {
-                br = spec[(o + kb) * 2];
-                bi = spec[(o + kb) * 2 + 1];
+    // rebuild Z = A + iB for m in [0, n) from the half spectra of the two levels of each pair.  kU
+    // elements per thread per round: every global load of a round is issued (clamped levels, so no load
+    // sits behind a branch) before any of them is used.
+    // With jhalf the half-slab j pass left row j of the full inverse as v0(j') + s v1(j'), j' = j mod
+    // jhalf, with v0 in row j' and v1 in row jhalf + j' (s = +1 on the first half of the rows, -1 on the
+    // second).
+    constexpr int kU = 4;
+    const int ne = n * P, nt = (int)blockDim.x;
+    const int jp = jhalf ? (j < jhalf ? j : j - jhalf) : j;
+    const T sg = j < jhalf ? T(1) : T(-1);
+    for (int idx0 = (int)threadIdx.x; idx0 < ne; idx0 += kU * nt) {
+        T x[kU][4], y[kU][4];  // (re, im) at ka, then at kb: row jp (x) and, with jhalf, row jhalf + jp (y)
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int idx = min(idx0 + u * nt, ne - 1), pp = idx % P, m = idx / P;
+            const int ka = min(kk0 + 2 * pp, nk2 - 1), kb = min(kk0 + 2 * pp + 1, nk2 - 1);
+            const int ms = m <= half ? m : n - m;
+            const long long o0 = ((long long)ms * g.NY + jp) * nk2;
+            x[u][0] = spec[(o0 + ka) * 2];
+            x[u][1] = spec[(o0 + ka) * 2 + 1];
+            x[u][2] = spec[(o0 + kb) * 2];
+            x[u][3] = spec[(o0 + kb) * 2 + 1];
+            if (jhalf) {
+                const long long o1 = ((long long)ms * g.NY + jhalf + jp) * nk2;
+                y[u][0] = spec[(o1 + ka) * 2];
+                y[u][1] = spec[(o1 + ka) * 2 + 1];
+                y[u][2] = spec[(o1 + kb) * 2];
+                y[u][3] = spec[(o1 + kb) * 2 + 1];
             }
         }
-        if (ms == 0 || ms == half) {  // self-conjugate wavenumbers: only the real part is kept
-            ai = T(0);
-            bi = T(0);
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int idx = idx0 + u * nt, pp = idx % P, m = idx / P;
+            const int ka = kk0 + 2 * pp, kb = ka + 1;
+            const int ms = m <= half ? m : n - m;
+            T ar = T(0), ai = T(0), br = T(0), bi = T(0);
+            if (jhalf) {
+                if (ka < nk2) {
+                    ar = x[u][0] + sg * y[u][0];
+                    ai = x[u][1] + sg * y[u][1];
+                }
+                if (kb < nk2) {
+                    br = x[u][2] + sg * y[u][2];
+                    bi = x[u][3] + sg * y[u][3];
+                }
+            } else {
+                if (ka < nk2) {
+                    ar = x[u][0];
+                    ai = x[u][1];
+                }
+                if (kb < nk2) {
+                    br = x[u][2];
+                    bi = x[u][3];
+                }
+            }
+            if (ms == 0 || ms == half) {  // self-conjugate wavenumbers: only the real part is kept
+                ai = T(0);
+                bi = T(0);
+            }
+            if (m > half) {  // conjugate mirror
+                ai = -ai;
+                bi = -bi;
+            }
+            if (idx < ne) {
+                re[m * P + pp] = ar - bi;
+                im[m * P + pp] = ai + br;
+            }
         }
-        if (m > half) {  // conjugate mirror
-            ai = -ai;
-            bi = -bi;
-        }
-        re[m * P + pp] = ar - bi;
-        im[m * P + pp] = ai + br;
     }
     __syncthreads();
     lds_fft_ip<T, kMaxBX>(re, im, rad, P, P, 1, twl, +1);
@@ -1010,6 +1062,67 @@ __device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs
     }
 }
 
+// Global <-> LDS copies of one kx slab: n complex values (re, im interleaved) at spec[base + 2 idx], in
+// the LDS planes at (idx / nk2) * P2 + idx % nk2.  Each thread issues kU global loads (clamped indices, so
+// no load sits behind a branch) before it writes any of them: a guarded load inside the loop made the
+// compiler wait for each one before issuing the next.
+constexpr int kSlabU = 8;
+template <typename T>
+__device__ __forceinline__ void slab_to_lds(const T *__restrict__ spec, long long base, int n, int nk2, int P2, T *re, T *im) {
+    const int nt = (int)blockDim.x, dk = nt % nk2, dy = nt / nk2;
+    int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
+    for (int i0 = (int)threadIdx.x; i0 < n; i0 += kSlabU * nt) {
+        T vr[kSlabU], vi[kSlabU];
+#pragma unroll
+        for (int u = 0; u < kSlabU; u++) {
+            const long long o = base + 2LL * min(i0 + u * nt, n - 1);
+            vr[u] = spec[o];
+            vi[u] = spec[o + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kSlabU; u++) {
+            if (i0 + u * nt < n) {
+                re[ky * P2 + kk] = vr[u];
+                im[ky * P2 + kk] = vi[u];
+            }
+            kk += dk;
+            ky += dy;
+            if (kk >= nk2) {
+                kk -= nk2;
+                ky++;
+            }
+        }
+    }
+}
+template <typename T>
+__device__ __forceinline__ void lds_to_slab(T *__restrict__ spec, long long base, int n, int nk2, int P2, const T *re, const T *im) {
+    const int nt = (int)blockDim.x, dk = nt % nk2, dy = nt / nk2, last = n / nk2 - 1;
+    int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
+    for (int i0 = (int)threadIdx.x; i0 < n; i0 += kSlabU * nt) {
+        T vr[kSlabU], vi[kSlabU];
+#pragma unroll
+        for (int u = 0; u < kSlabU; u++) {
+            const int e = min(ky, last) * P2 + kk;  // rows past the slab read its last row (not stored)
+            vr[u] = re[e];
+            vi[u] = im[e];
+            kk += dk;
+            ky += dy;
+            if (kk >= nk2) {
+                kk -= nk2;
+                ky++;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSlabU; u++) {
+            const int idx = i0 + u * nt;
+            if (idx < n) {
+                spec[base + 2LL * idx] = vr[u];
+                spec[base + 2LL * idx + 1] = vi[u];
+            }
+        }
+    }
+}
+
 // Q2 (fused): forward DFT along j, FP64 tridiagonal solve, backward DFT along j for one kx, with the
 // whole (j, k) slab of that kx resident in LDS
 template <typename T, bool TR, bool POW2 = true>
@@ -1021,24 +1134,7 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
     T *twl = (T *)(mean + 2 * nk2 + 1);
     load_twiddles(twl, twy, nj);
     const long long base = (long long)kx * nj * nk2 * 2;
-    // element idx = ky*nk2 + kk; (kk, ky) advance by the block size without dividing; unrolled so the
-    // global loads of several elements are in flight at once
-    const int dk = (int)blockDim.x % nk2, dy = (int)blockDim.x / nk2;
-    {
-        int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
-#pragma unroll 8
-        for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
-            const T vr = spec[base + 2 * (long long)idx], vi = spec[base + 2 * (long long)idx + 1];
-            re[ky * P2 + kk] = vr;
-            im[ky * P2 + kk] = vi;
-            kk += dk;
-            ky += dy;
-            if (kk >= nk2) {
-                kk -= nk2;
-                ky++;
-            }
-        }
-    }
+    slab_to_lds(spec, base, nj * nk2, nk2, P2, re, im);
     __syncthreads();
     lds_fft_ip<T, ypass_maxb<T>(), POW2>(re, im, rady, nk2, P2, 1, twl, -1);
     if ((int)threadIdx.x < 2 * nj) {
@@ -1050,19 +1146,7 @@ __device__ __forceinline__ void ypass_block(T *__restrict__ spec, int nk2, const
     }
     __syncthreads();
     lds_fft_ip<T, ypass_maxb<T>(), POW2>(re, im, rady, nk2, P2, 1, twl, +1);
-    {
-        int kk = (int)threadIdx.x % nk2, ky = (int)threadIdx.x / nk2;
-        for (int idx = threadIdx.x; idx < nj * nk2; idx += blockDim.x) {
-            spec[base + 2 * (long long)idx] = re[ky * P2 + kk];
-            spec[base + 2 * (long long)idx + 1] = im[ky * P2 + kk];
-            kk += dk;
-            ky += dy;
-            if (kk >= nk2) {
-                kk -= nk2;
-                ky++;
-            }
-        }
-    }
+    lds_to_slab(spec, base, nj * nk2, nk2, P2, re, im);
 }
 
 template <typename T>
@@ -1184,21 +1268,44 @@ __device__ __forceinline__ void ypass_half_block(const T *__restrict__ in, T *__
         twl[2 * e + 1] = T(twy[4 * e + 1]);
     }
     const long long base = (long long)kx * nj * nk2 * 2, second = 2LL * L * nk2;
-    for (int idx = threadIdx.x; idx < L * nk2; idx += blockDim.x) {
-        const int jp = idx / nk2, kk = idx - jp * nk2;
-        const long long o = base + 2LL * idx;
-        T ar, ai;
-        if (h == 0) {
-            ar = in[o] + in[o + second];
-            ai = in[o + 1] + in[o + second + 1];
-        } else {
-            const T dr = in[o] - in[o + second], di = in[o + 1] - in[o + second + 1];
-            ar = dr;
-            ai = di;
-            cmul(ar, ai, T(twy[2 * jp]), T(twy[2 * jp + 1]));
+    // kU elements per thread per round, every global load of a round issued (clamped indices) before any
+    // is used: a load behind the loop's bound check made the compiler wait for each in turn
+    constexpr int kU = 4;
+    const int ne = L * nk2, nt = (int)blockDim.x;
+    for (int idx0 = (int)threadIdx.x; idx0 < ne; idx0 += kU * nt) {
+        T a[kU][4];
+        double w[kU][2];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int idx = min(idx0 + u * nt, ne - 1), jp = idx / nk2;
+            const long long o = base + 2LL * idx;
+            a[u][0] = in[o];
+            a[u][1] = in[o + 1];
+            a[u][2] = in[o + second];
+            a[u][3] = in[o + second + 1];
+            if (h == 1) {
+                w[u][0] = twy[2 * jp];
+                w[u][1] = twy[2 * jp + 1];
+            }
         }
-        re[jp * P2 + kk] = ar;
-        im[jp * P2 + kk] = ai;
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int idx = idx0 + u * nt, jp = idx / nk2, kk = idx - jp * nk2;
+            T ar, ai;
+            if (h == 0) {
+                ar = a[u][0] + a[u][2];
+                ai = a[u][1] + a[u][3];
+            } else {
+                const T dr = a[u][0] - a[u][2], di = a[u][1] - a[u][3];
+                ar = dr;
+                ai = di;
+                cmul(ar, ai, T(w[u][0]), T(w[u][1]));
+            }
+            if (idx < ne) {
+                re[jp * P2 + kk] = ar;
+                im[jp * P2 + kk] = ai;
+            }
+        }
     }
     __syncthreads();
     lds_fft_pow2<T, half_maxb<T>()>(re, im, rh, nk2, P2, 1, twl, -1);
@@ -1210,13 +1317,29 @@ __device__ __forceinline__ void ypass_half_block(const T *__restrict__ in, T *__
     }
     __syncthreads();
     lds_fft_pow2<T, half_maxb<T>()>(re, im, rh, nk2, P2, 1, twl, +1);
-    for (int idx = threadIdx.x; idx < L * nk2; idx += blockDim.x) {
-        const int jp = idx / nk2, kk = idx - jp * nk2;
-        T vr = re[jp * P2 + kk], vi = im[jp * P2 + kk];
-        if (h == 1) cmul(vr, vi, T(twy[2 * jp]), -T(twy[2 * jp + 1]));
-        const long long o = base + 2LL * ((long long)h * L * nk2 + idx);
-        out[o] = vr;
-        out[o + 1] = vi;
+    for (int idx0 = (int)threadIdx.x; idx0 < ne; idx0 += kU * nt) {
+        T vr[kU], vi[kU];
+        double w[kU][2];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int idx = min(idx0 + u * nt, ne - 1), jp = idx / nk2, kk = idx - jp * nk2;
+            vr[u] = re[jp * P2 + kk];
+            vi[u] = im[jp * P2 + kk];
+            if (h == 1) {
+                w[u][0] = twy[2 * jp];
+                w[u][1] = twy[2 * jp + 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int idx = idx0 + u * nt;
+            if (h == 1) cmul(vr[u], vi[u], T(w[u][0]), -T(w[u][1]));
+            if (idx < ne) {
+                const long long o = base + 2LL * ((long long)h * L * nk2 + idx);
+                out[o] = vr[u];
+                out[o + 1] = vi[u];
+            }
+        }
     }
 }
 
