@@ -972,10 +972,14 @@ __device__ __forceinline__ void post_uvw_col(const Geo &g, const Cols &c, const 
     T *UP = (T *)a.f[0], *VP = (T *)a.f[1], *WP = (T *)a.f[2];
     const T *PI = (const T *)a.pi;
     T u = UP[p], v = VP[p], w = WP[p];
+    // every load of the column goes out before the arithmetic: the neighbours p - NYNZ, p - NZ, p - 1
+    // lie inside the array for every source column (isrc >= 3), used or not
+    const T pc = PI[p], pw = PI[p - g.NYNZ], psn = PI[p - NZ], pb = PI[p - 1];
+    const double mwk = c.mw[kc];
     if (pgf && j >= 1 && j <= g.NY - 2 && kc >= 1 && kc <= NZ - 2) {
-        u -= step * (T(g.dtx) * (PI[p] - PI[p - g.NYNZ]));
-        v -= step * (T(g.dty) * (PI[p] - PI[p - NZ]));
-        if (kc >= 2) w -= step * ((T(g.dtz) * T(c.mw[kc])) * (PI[p] - PI[p - 1]));
+        u -= step * (T(g.dtx) * (pc - pw));
+        v -= step * (T(g.dty) * (pc - psn));
+        if (kc >= 2) w -= step * ((T(g.dtz) * T(mwk)) * (pc - pb));
     }
     T coef = sponge_coef<T>(g, a, j);
     if (coef >= T(0) && kc >= 1) {
