@@ -120,13 +120,17 @@ def time_form(case, precision, steps, warmup, stage_streams):
                                             "wall_ms_per_step": [round(x, 4) for x in wall_ms]}
 
 
-def kernel_profile(g, steps, fused):
+def kernel_profile(g, steps, fused, shape=None):
     """Per-kernel HIP-event timings of the same step sequence the graph replays, launched eagerly, with
     the whole stage on one stream: the scalar half, which the step runs beside the pressure solve, would
-    otherwise share the GPU with the kernels it overlaps and its duration would not be its own."""
+    otherwise share the GPU with the kernels it overlaps and its duration would not be its own.  On one
+    stream the stage kernels take the library's one-stream launch shapes; `shape` (a dict), when given,
+    receives them."""
     from model_amd.core import DYN_TUNE_STAGE_STREAMS
     before = g.tune_get(DYN_TUNE_STAGE_STREAMS)
     g.tune_set(DYN_TUNE_STAGE_STREAMS, 1)
+    if shape is not None:
+        shape.update(launch_shape(g))
     g.timing(1)
     (g.step_fused if fused else g.step_reference)(steps)
     g.sync()
@@ -140,6 +144,13 @@ def kernel_profile(g, steps, fused):
         name, n, ms, units = line.split()
         rows.append((name, int(n), float(ms), float(units)))
     return rows
+
+
+def launch_shape(g):
+    """Rows per block and march length of the two stage kernels as the context launches them now."""
+    from model_amd.core import DYN_TUNE_MOMENTUM_TI, DYN_TUNE_MOMENTUM_TJ, DYN_TUNE_SCALAR_TI, DYN_TUNE_SCALAR_TJ
+    return {"momentum_rows": g.tune_get(DYN_TUNE_MOMENTUM_TJ), "momentum_march": g.tune_get(DYN_TUNE_MOMENTUM_TI),
+            "scalar_rows": g.tune_get(DYN_TUNE_SCALAR_TJ), "scalar_march": g.tune_get(DYN_TUNE_SCALAR_TI)}
 
 
 def roofline(rows, elt, moist):
@@ -425,10 +436,15 @@ def main():
             f"{res[name]['value']:.4g} gridpoint-steps/s")
     main_form = forms[0][0]
     g = handles[main_form]
-    rows = kernel_profile(g, args.profile_steps, fused=True)
+    step_shape, profile_shape = launch_shape(g), {}
+    rows = kernel_profile(g, args.profile_steps, fused=True, shape=profile_shape)
     for r in sorted(rows, key=lambda r: -r[2]):
         log(f"bench: kernel {r[0]:16s} launches {r[1]:5d} total {r[2]:9.3f} ms avg {r[2] / r[1] * 1e3:8.1f} us")
     roof = roofline(rows, 4 if main_form == "fp32" else 8, case.microphysics)
+    # the kernel durations come from the one-stream sequence and its launch shapes; the timed step runs
+    # the scalar half beside the pressure solve with its own
+    roof["measured_launch_shape"] = profile_shape
+    roof["step_launch_shape"] = step_shape
     if args.profile_only:
         log(f"bench: dominant kernel {roof['kernel']} {roof['achieved']} GB/s, traffic {roof['traffic']}")
         return 0

@@ -168,8 +168,11 @@ enum {
                                  only with one wavefront per column) */
     DYN_TUNE_MOMENTUM_TI = 1, /* columns one momentum block marches along i: 2 .. 512 */
     DYN_TUNE_SCALAR_TJ = 2,   /* j-rows per block of the scalar (theta, moisture) stage kernel: 2, 4, 8 or 12
-                                 (8 and 12 only with one wavefront per column) */
-    DYN_TUNE_SCALAR_TI = 3,   /* columns one scalar block marches along i: 2 .. 512 */
+                                 (8 and 12 only with one wavefront per column).  Default 8, or 4 for
+                                 small FP64 grids; FP32 with the stage on one stream (DYN_TUNE_STAGE_STREAMS
+                                 set to 1) takes 12 */
+    DYN_TUNE_SCALAR_TI = 3,   /* columns one scalar block marches along i: 2 .. 512.  With 12 rows the
+                                 default fills the device with one round of blocks (at least 16) */
     DYN_TUNE_PRESSURE_JPASS = 4, /* pressure j pass: 1 one LDS-resident kernel per kx (when the slab fits), 2 three
                                     kernels, 3 two LDS-resident kernels per kx on half slabs (even NY, when they fit) */
     DYN_TUNE_STAGE_STREAMS = 5,  /* fused step: 1 runs the whole stage on the context's stream; 2 runs a

@@ -1291,6 +1291,19 @@ bool fused_shape_ok(const Ctx *c, int param, int value) {
     return false;
 }
 
+namespace {
+// The FP32 scalar kernel on one stream (no pressure pass beside it): 12 rows per block (three waves per
+// SIMD at its register count, one block per CU) and march lengths that make one round of blocks fill the
+// device.  Measured at 256x256x64 moist: 92.7 us against 104.2 us for 8 rows x 16 columns, one-stream
+// step 0.956 against 0.990 ms.  Beside the pressure solve 8 rows stay faster (the scalar blocks must
+// leave room for the solve's).
+bool scalar_one_round(const Ctx *c) {
+    if (c->prec != DYN_FP32 || fused_waves(c) != 1 || c->pp.cus <= 0) return false;
+    if (c->tune[DYN_TUNE_STAGE_STREAMS] != 1) return false;  // one stream only when asked for
+    return fused_shape_ok(c, DYN_TUNE_SCALAR_TJ, 12);
+}
+}  // namespace
+
 int fused_shape(const Ctx *c, int param) {
     if (c->tune[param]) return c->tune[param];
     const bool f32 = c->prec == DYN_FP32;
@@ -1307,6 +1320,7 @@ int fused_shape(const Ctx *c, int param) {
             if (!f32 && param == DYN_TUNE_SCALAR_TJ && c->g.microphysics) return 2;
             return fused_shape_ok(c, param, 4) ? 4 : 2;
         }
+        if (param == DYN_TUNE_SCALAR_TJ && scalar_one_round(c)) return 12;
         return tj;
     case DYN_TUNE_MOMENTUM_TI: return fused_waves(c) == 2 ? 32 : 16;  // 512x512x80: 32 measured 3 % faster
     // The scalar kernel beside the pressure solve (fork after the momentum kernel): 24 columns, about 2 %
@@ -1316,6 +1330,11 @@ int fused_shape(const Ctx *c, int param) {
     // the later fork, where 16 measured best.
     case DYN_TUNE_SCALAR_TI:
         if (fused_waves(c) == 2) return 32;
+        if (scalar_one_round(c)) {  // as many i-chunks per 12-row band as leave every block one CU, at
+                                    // least 16 columns each (shorter marches spend more on the queue fill)
+            const int nbj = (c->g.ny - 6 + 11) / 12, per = std::max(1, c->pp.cus / nbj);
+            return std::min(512, std::max(16, (c->g.nx + per - 1) / per));
+        }
         {
             const int streams = fused_shape(c, DYN_TUNE_STAGE_STREAMS);
             return streams == 2 || streams == 4 ? 24 : 16;

@@ -78,6 +78,7 @@ struct PressurePlan {
     bool half_y = false;               // ... or in two per kx on half slabs (even nj; one radix-2 split)
     void *spec2 = nullptr;             // output of the half-slab j pass ([kx][j][kk] complex, T)
     size_t lds_max = 0;
+    int cus = 0;  // compute units of the device
 };
 
 // per-kernel HIP-event timing on the context stream (eager launches only)

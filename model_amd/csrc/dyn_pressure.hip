@@ -1820,6 +1820,7 @@ int pressure_init(Ctx *c) {
     DYN_HIP(hipGetDevice(&dev));
     DYN_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     pp.lds_max = (size_t)lds;
+    DYN_HIP(hipDeviceGetAttribute(&pp.cus, hipDeviceAttributeMultiprocessorCount, dev));
     // the fused j pass keeps a whole (j, k) slab of one kx in LDS when it fits (FP32 at 256x64, FP64 up to
     // about 192x40) and one thread per real sequence of the solve exists; otherwise the j transforms and
     // the solve run as three kernels

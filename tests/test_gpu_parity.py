@@ -417,6 +417,23 @@ def test_fused_launch_shapes_bitwise(precision, shape):
         assert np.array_equal(a.get_field(f, role), b.get_field(f, role)), ids.FIELD_NAMES[f]
 
 
+def test_one_stream_scalar_shape():
+    """FP32 on one stream: the scalar kernel takes 12 rows per block and a march length that fills the
+    device in one round of blocks (at least 16 columns); beside the pressure solve it keeps 8 rows."""
+    from model_amd import core
+    g = DynCore(configs.supercell(nx=256, ny=256, nz=20), precision=DYN_FP32)
+    assert g.tune_get(core.DYN_TUNE_SCALAR_TJ) == 8
+    g.tune_set(core.DYN_TUNE_STAGE_STREAMS, 1)
+    assert g.tune_get(core.DYN_TUNE_SCALAR_TJ) == 12
+    ti = g.tune_get(core.DYN_TUNE_SCALAR_TI)
+    assert ti >= 16 and ((250 + 11) // 12) * ((256 + ti - 1) // ti) <= 256, ti
+    g.close()
+    small = DynCore(configs.supercell(nx=40, ny=36, nz=30), precision=DYN_FP32)
+    small.tune_set(core.DYN_TUNE_STAGE_STREAMS, 1)
+    assert small.tune_get(core.DYN_TUNE_SCALAR_TI) == 16
+    small.close()
+
+
 @pytest.mark.parametrize("precision", [0, DYN_FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("moist", [0, 1], ids=["dry", "moist"])
 def test_step_calls_of_any_length_bitwise(precision, moist):
