@@ -900,20 +900,29 @@ int dyn_adjoint_record(dyn_ctx *ctx, int nsteps, int every) {
     // forward pass with a snapshot every `every` steps and one of the final state; the last segment's
     // stage states are recorded on the way, so the reverse sweep starts without a recomputation
     const int last0 = (nseg - 1) * s->every;
-    for (int n = 0; n < nsteps; n++) {
+    for (int n = 0; n < nsteps;) {
         if (n % s->every == 0) {
             rc = copy_state(c, &s->snap[(size_t)(n / s->every) * 12], nullptr, true);
             if (rc) return rc;
+        }
+        if (n < last0) {
+            // a segment before the last records nothing: its steps replay as dyn_step's graphs (the
+            // same steps bit for bit), which leave the roles apart for the next snapshot
+            rc = dyn_step(reinterpret_cast<dyn_ctx *>(c), s->every);
+            if (rc) return rc;
+            n += s->every;
+            continue;
         }
         int local = n - last0;
         // M and current share a buffer between the steps (the snapshots copy each role's values, the
         // same whether or not they share); the last step separates them
         const bool share = n < nsteps - 1;
-        rc = n >= last0 ? step_with_hook(c, record_hook, &local, share) : step_with_hook(c, nullptr, nullptr, share);
+        rc = step_with_hook(c, record_hook, &local, share);
         if (rc) {
             unshare_roles(c);
             return rc;
         }
+        n++;
     }
     s->rec_seg[0] = nseg - 1;
     rc = copy_state(c, &s->snap[(size_t)nseg * 12], nullptr, true);

@@ -1526,10 +1526,11 @@ size_t y_pass_lds(int nj, int nk2, size_t elt) {
 // (`fill` false), which keeps the row segments long.
 int x_level_chunk(const PressurePlan &pp, const Geo &g, const Radix &rx, size_t elt, bool fill = true) {
     int kc = (int)((kLdsBudget - 2 * (size_t)pp.ni * elt) / (2 * (size_t)pp.ni * elt)) * 2;
-    // about 512 blocks per pass in FP32; FP64 takes half the levels per block (256x256x64: 1.909 -> 1.869
-    // ms per step with 16 levels instead of 30; FP32 is slower with fewer than 32)
+    // about 512 blocks per pass in FP32; FP64 takes half the levels per block down to 16 (256x256x64:
+    // 1.909 -> 1.869 ms per step with 16 levels instead of 30; FP32 is slower with fewer than 32, and
+    // FP64 at 192x192x40 with 6 instead of 14)
     int want = fill ? std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1) : (pp.nk2 + 1) & ~1;
-    if (fill && elt == 8) want = std::max(2, (want / 2) & ~1);
+    if (fill && elt == 8 && want > 16) want = std::max(16, (want / 2) & ~1);
     kc = std::max(2, std::min(kc, want));
     while (kc > 2 && max_butterflies(rx, kc / 2, kThreadsX) > kMaxBX) kc -= 2;
     return kc;
