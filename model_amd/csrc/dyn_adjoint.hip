@@ -94,6 +94,26 @@ struct AdjCommon {
     __device__ T AV(int ii, int jj, int kk) const { return inb(ii, jj, kk) ? step * a.PV[X(ii, jj, kk)] : T(0); }
     __device__ T ATH(int ii, int jj, int kk) const { return inb(ii, jj, kk) ? step * a.PTH[X(ii, jj, kk)] : T(0); }
     __device__ T AW(int ii, int jj, int kk) const { return inb(ii, jj, kk) && kk >= 2 ? step * a.PW[X(ii, jj, kk)] : T(0); }
+    // the same values from an unconditional load at the index clamped into the array, then a select
+    __device__ long long XC(int ii, int jj, int kk) const {
+        return IX(min(max(ii, 0), g.NX - 1), min(max(jj, 0), g.NY - 1), min(max(kk, 0), g.NZ - 1));
+    }
+    __device__ T AUc(int ii, int jj, int kk) const {
+        const T v = step * a.PU[XC(ii, jj, kk)];
+        return inb(ii, jj, kk) ? v : T(0);
+    }
+    __device__ T AVc(int ii, int jj, int kk) const {
+        const T v = step * a.PV[XC(ii, jj, kk)];
+        return inb(ii, jj, kk) ? v : T(0);
+    }
+    __device__ T AWc(int ii, int jj, int kk) const {
+        const T v = step * a.PW[XC(ii, jj, kk)];
+        return inb(ii, jj, kk) && kk >= 2 ? v : T(0);
+    }
+    __device__ T ATHc(int ii, int jj, int kk) const {
+        const T v = step * a.PTH[XC(ii, jj, kk)];
+        return inb(ii, jj, kk) ? v : T(0);
+    }
     __device__ T DTZ(int kk) const { return T(g.dtz) * T(c.mu[kk]); }
     // tend(k) -= DTZ(k) (rhow[k+1] top(k) - rhow[k] top(k-1)) one_d_rhou[k]: adjoint of the top face kf
     __device__ T zcoef_lo(int kf) const { return -DTZ(kf) * T(c.rhow[kf + 1]) * T(c.one_d_rhou[kf]); }
@@ -120,84 +140,151 @@ __global__ void __launch_bounds__(256) k_adj_gather(Geo g, Cols c, AdvAdjArgs<T>
     const T dt = T(g.dt);
     const bool krow = k >= 1 && k <= NZ - 2, jrow = j >= jl && j < jh, icol = i >= il && i < ih;
     T gu = T(0), gv = T(0), gw = T(0), gth = T(0);
+    // The three face loops run their six faces unrolled: every load of a loop is issued (clamped
+    // indices) before any is used, and a face outside the loop's range contributes nothing (a select, so
+    // whatever the face arrays hold there never enters).  A scheduling barrier after each loop keeps its
+    // loads from being hoisted into the one before, which would raise the register count.
+    const int jc = min(max(j, 0), g.NY - 1), kc = min(max(k, 1), NZ - 2);
     // east faces: stencils of u, v, theta; u as its own advecting velocity
     if (jrow && krow) {
-        for (int e = max(i - 3, il - 1); e <= min(i + 2, ih - 1); e++) {
-            const int d = i - e + 2;
-            gu += stencil(FEU, e, j, k, c6[d], u5[d]);
-            gv += stencil(FEV, e, j, k, c6[d], u5[d]);
-            gth += stencil(FET, e, j, k, c6[d], u5[d]);
-            if (e == i - 1 || e == i) gu += T(0.5) * F(FEU, 2, e, j, k);
+        T aw[6][3], sg[6][3], ge[2];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const long long o = IX(min(max(i - 3 + q, 0), g.NX - 1), jc, kc);
+            aw[q][0] = ((const T *)fa.f[2 * FEU])[o];
+            aw[q][1] = ((const T *)fa.f[2 * FEV])[o];
+            aw[q][2] = ((const T *)fa.f[2 * FET])[o];
+            sg[q][0] = T(fa.sgn[FEU][o]);
+            sg[q][1] = T(fa.sgn[FEV][o]);
+            sg[q][2] = T(fa.sgn[FET][o]);
+        }
+        ge[0] = F(FEU, 2, max(i - 1, 0), j, k);
+        ge[1] = F(FEU, 2, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int e = i - 3 + q, d = 5 - q;  // d = i - e + 2
+            if (e >= il - 1 && e <= ih - 1) {
+                gu += aw[q][0] * (c6[d] + sg[q][0] * u5[d]);
+                gv += aw[q][1] * (c6[d] + sg[q][1] * u5[d]);
+                gth += aw[q][2] * (c6[d] + sg[q][2] * u5[d]);
+                if (q == 2 || q == 3) gu += T(0.5) * ge[q - 2];
+            }
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
     if (krow) {
-        const int e = i - 1;
+        const int e = i - 1, ec = max(e, 0), jp = min(j + 1, g.NY - 1), jm = max(j - 1, 0);
+        const T g0 = F(FEV, 2, ec, j, k), g1 = F(FEV, 2, ec, jp, k), g2 = F(FET, 2, ec, j, k);
+        const T av0 = m.AVc(i, j, k), av1 = m.AVc(i - 1, j, k), av2 = m.AVc(i, j + 1, k), av3 = m.AVc(i - 1, j + 1, k);
+        const T au0 = m.AUc(i, j, k), au1 = m.AUc(i + 1, j, k), au2 = m.AUc(i, j - 1, k), au3 = m.AUc(i + 1, j - 1, k);
+        const T f0 = T(c.f[j]), fp = T(c.f[jp]), fm = T(c.f[jm]);
         if (e >= il - 1 && e <= ih - 1) {  // U(i, j) advects v at (e, j) and (e, j+1) and theta at e
-            if (j >= jl && j < jh) gu += T(0.5) * F(FEV, 2, e, j, k);
-            if (j + 1 >= jl && j + 1 < jh) gu += T(0.5) * F(FEV, 2, e, j + 1, k);
-            if (jrow) gu += F(FET, 2, e, j, k);
+            if (j >= jl && j < jh) gu += T(0.5) * g0;
+            if (j + 1 >= jl && j + 1 < jh) gu += T(0.5) * g1;
+            if (jrow) gu += g2;
         }
         // Coriolis: v tend -= dt f (U(i',j') + U(i'+1,j') + U(i',j'-1) + U(i'+1,j'-1)) / 4, u tend += dt f (V ...) / 4
-        gu -= T(0.25) * dt *
-              (T(c.f[j]) * (m.AV(i, j, k) + m.AV(i - 1, j, k)) + (j + 1 < g.NY ? T(c.f[j + 1]) * (m.AV(i, j + 1, k) + m.AV(i - 1, j + 1, k)) : T(0)));
-        gv += T(0.25) * dt *
-              (T(c.f[j]) * (m.AU(i, j, k) + m.AU(i + 1, j, k)) + (j >= 1 ? T(c.f[j - 1]) * (m.AU(i, j - 1, k) + m.AU(i + 1, j - 1, k)) : T(0)));
+        gu -= T(0.25) * dt * (f0 * (av0 + av1) + (j + 1 < g.NY ? fp * (av2 + av3) : T(0)));
+        gv += T(0.25) * dt * (f0 * (au0 + au1) + (j >= 1 ? fm * (au2 + au3) : T(0)));
     }
+    __builtin_amdgcn_sched_barrier(0);
     // north faces
     if (icol && krow) {
-        for (int f = max(j - 3, jl - 1); f <= min(j + 2, jh - 1); f++) {
-            const int d = j - f + 2;
-            gu += stencil(FNU, i, f, k, c6[d], u5[d]);
-            gv += stencil(FNV, i, f, k, c6[d], u5[d]);
-            gth += stencil(FNT, i, f, k, c6[d], u5[d]);
-            if (f == j - 1 || f == j) gv += T(0.5) * F(FNV, 2, i, f, k);
+        T aw[6][3], sg[6][3], gn[2];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const long long o = IX(i, min(max(j - 3 + q, 0), g.NY - 1), kc);
+            aw[q][0] = ((const T *)fa.f[2 * FNU])[o];
+            aw[q][1] = ((const T *)fa.f[2 * FNV])[o];
+            aw[q][2] = ((const T *)fa.f[2 * FNT])[o];
+            sg[q][0] = T(fa.sgn[FNU][o]);
+            sg[q][1] = T(fa.sgn[FNV][o]);
+            sg[q][2] = T(fa.sgn[FNT][o]);
+        }
+        gn[0] = F(FNV, 2, i, max(j - 1, 0), k);
+        gn[1] = F(FNV, 2, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int f = j - 3 + q, d = 5 - q;  // d = j - f + 2
+            if (f >= jl - 1 && f <= jh - 1) {
+                gu += aw[q][0] * (c6[d] + sg[q][0] * u5[d]);
+                gv += aw[q][1] * (c6[d] + sg[q][1] * u5[d]);
+                gth += aw[q][2] * (c6[d] + sg[q][2] * u5[d]);
+                if (q == 2 || q == 3) gv += T(0.5) * gn[q - 2];
+            }
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
     if (krow) {
-        const int f = j - 1;
+        const int f = j - 1, fc = max(f, 0), ip = min(i + 1, g.NX - 1);
+        const T g0 = F(FNU, 2, i, fc, k), g1 = F(FNU, 2, ip, fc, k), g2 = F(FNT, 2, i, fc, k);
         if (f >= jl - 1 && f <= jh - 1) {  // V(i, j) advects u at (i, f) and (i+1, f) and theta at (i, f)
-            if (i >= il && i < ih) gv += T(0.5) * F(FNU, 2, i, f, k);
-            if (i + 1 >= il && i + 1 < ih) gv += T(0.5) * F(FNU, 2, i + 1, f, k);
-            if (icol) gv += F(FNT, 2, i, f, k);
+            if (i >= il && i < ih) gv += T(0.5) * g0;
+            if (i + 1 >= il && i + 1 < ih) gv += T(0.5) * g1;
+            if (icol) gv += g2;
         }
     }
-    // top faces
+    __builtin_amdgcn_sched_barrier(0);
+    // top faces kf = k - 2 .. k + 1 (q = kf - k + 2), unrolled like the loops above; kf = 1 and NZ - 2
+    // are second-order faces
     if (icol && jrow) {
-        for (int kf = max(k - 2, 1); kf <= min(k + 1, NZ - 2); kf++) {
+        T aw[4][3], sg[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long long o = IX(i, j, min(max(k - 2 + q, 1), NZ - 2));
+            aw[q][0] = ((const T *)fa.f[2 * FZU])[o];
+            aw[q][1] = ((const T *)fa.f[2 * FZV])[o];
+            aw[q][2] = ((const T *)fa.f[2 * FZT])[o];
+            sg[q][0] = T(fa.sgn[FZU][o]);
+            sg[q][1] = T(fa.sgn[FZV][o]);
+            sg[q][2] = T(fa.sgn[FZT][o]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int kf = k - 2 + q;
+            if (kf < 1 || kf > NZ - 2) continue;
             if (kf == 1 || kf == NZ - 2) {
                 if (k == kf || k == kf + 1) {
-                    gu += T(0.5) * F(FZU, 0, i, j, kf);
-                    gv += T(0.5) * F(FZV, 0, i, j, kf);
-                    gth += T(0.5) * F(FZT, 0, i, j, kf);
+                    gu += T(0.5) * aw[q][0];
+                    gv += T(0.5) * aw[q][1];
+                    gth += T(0.5) * aw[q][2];
                 }
             } else {
-                const int d = k - kf + 1;
-                gu += stencil(FZU, i, j, kf, c4[d], u3[d]);
-                gv += stencil(FZV, i, j, kf, c4[d], u3[d]);
-                gth += stencil(FZT, i, j, kf, c4[d], u3[d]);
+                const int d = 3 - q;  // d = k - kf + 1
+                gu += aw[q][0] * (c4[d] + sg[q][0] * u3[d]);
+                gv += aw[q][1] * (c4[d] + sg[q][1] * u3[d]);
+                gth += aw[q][2] * (c4[d] + sg[q][2] * u3[d]);
             }
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
     {
         const int kf = k - 1;  // W(k) advects at the top face of level k-1
+        const int kfc = min(max(kf, 0), NZ - 1), ip = min(i + 1, g.NX - 1), jp = min(j + 1, g.NY - 1);
+        const int kc0 = min(max(k, 1), NZ - 1), kp = min(k + 1, NZ - 1);
+        const T g0 = F(FZU, 2, i, j, kfc), g1 = F(FZU, 2, ip, j, kfc), g2 = F(FZV, 2, i, j, kfc), g3 = F(FZV, 2, i, jp, kfc);
+        const T g4 = F(FZT, 2, i, j, kfc);
+        const T ath0 = m.ATHc(i, j, k - 1), ath1 = m.ATHc(i, j, k), aw0 = m.AWc(i, j, k), aw1 = m.AWc(i, j, kp);
+        const double rwk = c.rhow[kc0], tbk = c.tb[kc0], tbm = c.tb[kc0 - 1], odm = c.one_d_rhou[kc0 - 1], odk = c.one_d_rhou[kc0];
+        const double tbvk = c.tbv[min(max(k, 0), NZ - 1)];
         if (kf >= 1 && kf <= NZ - 2) {
             if (jrow) {
-                if (i >= il && i < ih) gw += T(0.5) * F(FZU, 2, i, j, kf);
-                if (i + 1 >= il && i + 1 < ih) gw += T(0.5) * F(FZU, 2, i + 1, j, kf);
+                if (i >= il && i < ih) gw += T(0.5) * g0;
+                if (i + 1 >= il && i + 1 < ih) gw += T(0.5) * g1;
             }
             if (icol) {
-                if (j >= jl && j < jh) gw += T(0.5) * F(FZV, 2, i, j, kf);
-                if (j + 1 >= jl && j + 1 < jh) gw += T(0.5) * F(FZV, 2, i, j + 1, kf);
+                if (j >= jl && j < jh) gw += T(0.5) * g2;
+                if (j + 1 >= jl && j + 1 < jh) gw += T(0.5) * g3;
             }
-            if (icol && jrow) gw += F(FZT, 2, i, j, kf);
+            if (icol && jrow) gw += g4;
         }
         if (icol && jrow && k >= 1) {
             // Z_ADVECT_BASE (advection.cpp:16-18): W(k) enters the base-state term of levels k-1 and k
-            const T coef = T(0.5) * T(c.rhow[k]) * (T(c.tb[k]) - T(c.tb[k - 1]));
-            gw -= coef * (m.DTZ(k - 1) * T(c.one_d_rhou[k - 1]) * m.ATH(i, j, k - 1) + m.DTZ(k) * T(c.one_d_rhou[k]) * m.ATH(i, j, k));
+            const T coef = T(0.5) * T(rwk) * (T(tbk) - T(tbm));
+            gw -= coef * (m.DTZ(k - 1) * T(odm) * ath0 + m.DTZ(k) * T(odk) * ath1);
         }
         // buoyancy (advection.cpp:329-337): WP(k') = WM + step dt g (th(k')/tbv(k') + th(k'-1)/tbv(k'-1)) / 2
-        if (icol && jrow) gth += dt * T(kGrav) * T(0.5) * (m.AW(i, j, k) + (k + 1 < NZ ? m.AW(i, j, k + 1) : T(0))) / T(c.tbv[k]);
+        if (icol && jrow) gth += dt * T(kGrav) * T(0.5) * (aw0 + (k + 1 < NZ ? aw1 : T(0))) / T(tbvk);
     }
     const long long p = IX(i, j, k);
     a.CU[p] += gu;
@@ -209,8 +296,9 @@ __global__ void __launch_bounds__(256) k_adj_gather(Geo g, Cols c, AdvAdjArgs<T>
 // Pass 1 with every load unconditional: indices are pulled into the array (IXC) and only the stores are
 // guarded, so a wavefront issues its loads together instead of waiting at each guarded block.  Capped at
 // 96 VGPRs (five waves per SIMD; the uncapped form holds 120 and four), it measured 3 % faster than the
-// guarded form on the 192x192x40 adjoint.  The same treatment of pass 2 needs 229 VGPRs (two waves per
-// SIMD) and was slower, so pass 2 keeps its guarded blocks.
+// guarded form on the 192x192x40 adjoint.  The same treatment of pass 2 as a whole needs 229 VGPRs (two
+// waves per SIMD) and was slower; pass 2 batches its loads section by section instead, with a scheduling
+// barrier between sections (83 VGPRs in FP64; 154 -> about 120 us at 192x192x40).
 #define IXC(i, j, k) IX(min(max((i), 0), g.NX - 1), min(max((j), 0), g.NY - 1), min(max((k), 0), g.NZ - 1))
 
 template <typename T>
