@@ -40,6 +40,30 @@ def test_ensemble_matches_members_alone(precision, graph, groups):
     ens.close()
 
 
+@pytest.mark.parametrize("precision", [DYN_FP32, 0], ids=["fp32", "fp64"])
+def test_ensemble_with_a_prime_length_member(precision):
+    """A member whose i transform length has a prime factor above 5 (ni = 46 = 2 x 23, nj = 38 = 2 x 19)
+    cannot join the batched pressure solve, so the ensemble solves each member's pressure on its own and
+    batches the rest of the step: every member still gives the bits of its solo run."""
+    sizes = [(48, 64), (46, 38), (64, 48)]
+    cases = [configs.vortex_in_shear(nx=n, ny=m, nz=20, seed_offset=30 + k) for k, (n, m) in enumerate(sizes)]
+    alone = [DynCore(c, precision=precision) for c in cases]
+    for g in alone:
+        g.step(4)
+        g.sync()
+    members = [DynCore(c, precision=precision) for c in cases]
+    ens = Ensemble(members)
+    ens.step(3)
+    ens.step(1, graph=False)
+    ens.sync()
+    for a, b in zip(alone, members):
+        for f in FIELDS:
+            for role in (ids.ROLE_M, ids.ROLE_CUR, ids.ROLE_P):
+                x, y = a.get_field(f, role), b.get_field(f, role)
+                assert np.array_equal(x, y), (f, role, np.abs(x - y).max())
+    ens.close()
+
+
 def test_ensemble_autotune_keeps_state_and_values():
     cases = _cases()
     alone = [DynCore(c, precision=DYN_FP32) for c in cases]
