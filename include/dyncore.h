@@ -172,7 +172,7 @@ enum {
                                  small FP64 grids; FP32 with the stage on one stream (DYN_TUNE_STAGE_STREAMS
                                  set to 1) takes 12 */
     DYN_TUNE_SCALAR_TI = 3,   /* columns one scalar block marches along i: 2 .. 512.  With 12 rows the
-                                 default fills the device with one round of blocks (at least 16) */
+                                 default fills the device with one wave of blocks (at least 16) */
     DYN_TUNE_PRESSURE_JPASS = 4, /* pressure j pass: 1 one LDS-resident kernel per kx (when the slab fits), 2 three
                                     kernels, 3 two LDS-resident kernels per kx on half slabs (even NY, when they fit) */
     DYN_TUNE_STAGE_STREAMS = 5,  /* fused step: 1 runs the whole stage on the context's stream; 2 runs a

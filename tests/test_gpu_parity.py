@@ -419,7 +419,7 @@ def test_fused_launch_shapes_bitwise(precision, shape):
 
 def test_one_stream_scalar_shape():
     """FP32 on one stream: the scalar kernel takes 12 rows per block and a march length that fills the
-    device in one round of blocks (at least 16 columns); beside the pressure solve it keeps 8 rows."""
+    device in one wave of blocks (at least 16 columns); beside the pressure solve it keeps 8 rows."""
     from model_amd import core
     g = DynCore(configs.supercell(nx=256, ny=256, nz=20), precision=DYN_FP32)
     assert g.tune_get(core.DYN_TUNE_SCALAR_TJ) == 8
