@@ -1176,6 +1176,13 @@ void orc_get_turbulence(const orc_state *s, int which, double *out) {
     memcpy(out, src[which], sizeof(double) * (size_t)s->NX * s->NYNZ);
 }
 
+/* Kessler's saturation flags set directly, one byte per point [NX][NY][NZ] (turbulence on), so tests can
+ * reach the saturated branch of the Brunt-Vaisala frequency with states of their choosing */
+void orc_set_saturated(orc_state *s, const unsigned char *flags) {
+    if (!s->saturated) return;
+    memcpy(s->saturated, flags, (size_t)s->NX * s->NYNZ);
+}
+
 /* ------------------------------------------------------------------------------------------------
  * Buffer rotation, solver.cpp:229-263 and microphysics.cpp:642-669
  * ----------------------------------------------------------------------------------------------*/

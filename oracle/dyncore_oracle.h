@@ -92,6 +92,8 @@ void orc_apply_scalar_diffusion(orc_state *s, double step, int il, int ih, int j
 void orc_apply_moisture_diffusion(orc_state *s, double step, int il, int ih, int jl, int jh);
 /* 0 Kmix, 1 KHmix, 2-4 u/v/w friction, 5-8 theta/qv/qc/qr diffusion tendencies, [NX][NY][NZ] */
 void orc_get_turbulence(const orc_state *s, int which, double *out);
+/* Kessler's saturation flags [NX][NY][NZ] (0 or 1) set directly, for tests of the moist branch */
+void orc_set_saturated(orc_state *s, const unsigned char *flags);
 
 /* ---- pieces exposed so tests can pin them against mathematics ---- */
 /* in-place complex DFT of length n over interleaved (re,im) data; sign -1 forward, +1 backward,

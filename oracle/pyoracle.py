@@ -97,6 +97,7 @@ def lib(variant: str = "parity"):
     for fn in ("orc_apply_velocity_diffusion", "orc_apply_scalar_diffusion", "orc_apply_moisture_diffusion"):
         getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4
     L.orc_get_turbulence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.orc_set_saturated.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     _LIBS[variant] = L
     return L
 
@@ -160,6 +161,13 @@ class Oracle:
         ls = None if landsea is None else np.ascontiguousarray(landsea, dtype=np.float64)
         self._landsea = ls
         self.L.orc_set_turbulence(self.h, int(on), None if ls is None else _ptr(ls))
+
+    def set_saturated(self, flags):
+        """Kessler's saturation flags (nx, ny, nz; nonzero = saturated), as the next calculate_diff_tend
+        reads them; turbulence must be on."""
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert f.shape == (self.case.nx, self.case.ny, self.case.nz)
+        self.L.orc_set_saturated(self.h, f.ctypes.data)
 
     def get_turbulence(self, which):
         """0 Kmix, 1 KHmix, 2-4 u/v/w friction, 5-8 theta/qv/qc/qr diffusion tendencies, 9 saturation flag."""
