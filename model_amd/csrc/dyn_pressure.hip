@@ -907,13 +907,70 @@ constexpr int kThomasChunk = 8;  // levels per chunk of the Thomas sweeps
 
 // Forward elimination and back substitution of one real sequence (the real or the imaginary part) of one
 // (kx,ky) column whose values sit at x[kk*xs]; FP64 arithmetic with the pivots and multipliers of the
-// tables; the (0,0) column keeps its eliminated values in the FP64 scratch mean[2*kk] instead of the
+// tables; the (0,0) column (m0) keeps its eliminated values in the FP64 scratch mean[2*kk] instead of the
 // storage type (pressure.cpp:837-871).  Level k = kk+1; acoef[1] = 0, so the first level needs no
 // special case.
-// The sweeps walk kTC levels at a time.  Full chunks are straight-line code: the chunk's values, its
-// acoef entries and the next chunk's table entries (clamped indices, so every load is unconditional) are
-// all issued before the chain starts, and only the last partial chunk takes guarded steps.  Branchy
-// chunks made the compiler wait on every load as soon as it was issued, prefetch included.
+// The sweeps walk kTC levels at a time, and the table entries of the next chunk are loaded while the
+// current chunk is eliminated, into the second of two register sets that take turns: a loop that copied
+// the prefetched set into the current one at its end had to wait there for the loads just issued, so
+// every chunk waited a whole memory latency.  Every load is unconditional (clamped indices); only the
+// last partial chunk guards its steps.  The (0,0) column differs only in where its values are kept (a
+// per-lane choice of store and of load): an instance of its own would diverge from the other lanes of
+// its wavefront and run the whole chain twice.
+template <int kTC>
+__device__ __forceinline__ void thomas_tab_up(const double *Tab, long long fs, int k0, int hi, double (&v)[kTC]) {
+#pragma unroll
+    for (int t = 0; t < kTC; t++) v[t] = Tab[min(k0 + t, hi) * fs];
+}
+template <int kTC>
+__device__ __forceinline__ void thomas_tab_down(const double *Tab, long long fs, int k0, double (&v)[kTC]) {
+#pragma unroll
+    for (int t = 0; t < kTC; t++) v[t] = Tab[max(k0 - t, 0) * fs];
+}
+// p_kk = (x_kk / (ni nj) - a_k p_{kk-1}) z_kk for the chunk's levels kk = k0 .. k0+kTC-1 (< N unless FULL)
+template <typename T, bool FULL, int kTC>
+__device__ __forceinline__ void thomas_fwd_chunk(const ThomasArgs &ta, const double (&z)[kTC], int k0, int N, double size2d, T *x,
+                                                 int xs, bool m0, double *mean, double &p) {
+    double v[kTC], ac[kTC];
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = FULL ? k0 + t : min(k0 + t, N - 1);
+        v[t] = double(x[kk * xs]);
+        ac[t] = ta.acoef[kk + 1];
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        if (FULL || k0 + t < N) p = (v[t] / size2d - ac[t] * p) * z[t];
+        v[t] = p;
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        if (FULL || k0 + t < N) {
+            if (m0) mean[2 * (k0 + t)] = v[t];
+            else x[(k0 + t) * xs] = T(v[t]);
+        }
+    }
+}
+// x_kk = y_kk - F_kk x_{kk+1} for kk = k0 .. k0-kTC+1 (>= 0 unless FULL)
+template <typename T, bool FULL, int kTC>
+__device__ __forceinline__ void thomas_back_chunk(const double (&f)[kTC], int k0, T *x, int xs, bool m0, const double *mean,
+                                                  double &p) {
+    double v[kTC];
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = FULL ? k0 - t : max(k0 - t, 0);
+        const double vm = mean[2 * kk], vx = double(x[kk * xs]);
+        v[t] = m0 ? vm : vx;
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        if (FULL || k0 - t >= 0) p = v[t] - f[t] * p;
+        v[t] = p;
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++)
+        if (FULL || k0 - t >= 0) x[(k0 - t) * xs] = T(v[t]);
+}
 template <typename T, int kTC = kThomasChunk>
 __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, double *mean) {
     const int N = ta.nk - 2;
@@ -921,78 +978,84 @@ __device__ void thomas_part(const ThomasArgs &ta, int kx, int ky, T *x, int xs, 
     const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
     const bool m0 = (kx == 0 && ky == 0);
-    double cur[kTC], nxt[kTC], v[kTC];
-#pragma unroll
-    for (int t = 0; t < kTC; t++) nxt[t] = Z[min(t, N - 1) * fs];
-    double p = 0.0;
+    double A[kTC], B[kTC], p = 0.0;
+    // full chunks two at a time (one path through the loop, so the loads of the set not in use stay in
+    // flight across its back edge), then the levels left
+    thomas_tab_up<kTC>(Z, fs, 0, N - 1, A);
     int k0 = 0;
-    for (; k0 + kTC <= N; k0 += kTC) {
-        double ac[kTC];
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            cur[t] = nxt[t];
-            nxt[t] = Z[min(k0 + kTC + t, N - 1) * fs];
-            v[t] = double(x[(k0 + t) * xs]);
-            ac[t] = ta.acoef[k0 + t + 1];
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            p = (v[t] / size2d - ac[t] * p) * cur[t];
-            v[t] = p;
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            if (m0) mean[2 * (k0 + t)] = v[t];
-            else x[(k0 + t) * xs] = T(v[t]);
-        }
+    for (; k0 + 2 * kTC <= N; k0 += 2 * kTC) {
+        thomas_tab_up<kTC>(Z, fs, k0 + kTC, N - 1, B);
+        thomas_fwd_chunk<T, true, kTC>(ta, A, k0, N, size2d, x, xs, m0, mean, p);
+        thomas_tab_up<kTC>(Z, fs, k0 + 2 * kTC, N - 1, A);
+        thomas_fwd_chunk<T, true, kTC>(ta, B, k0 + kTC, N, size2d, x, xs, m0, mean, p);
     }
-#pragma unroll
-    for (int t = 0; t < kTC; t++) {
-        const int kk = k0 + t;
-        if (kk < N) {
-            p = (double(x[kk * xs]) / size2d - ta.acoef[kk + 1] * p) * nxt[t];
-            if (m0) mean[2 * kk] = p;
-            else x[kk * xs] = T(p);
-        }
+    if (k0 < N) {
+        thomas_tab_up<kTC>(Z, fs, k0 + kTC, N - 1, B);
+        if (k0 + kTC <= N) thomas_fwd_chunk<T, true, kTC>(ta, A, k0, N, size2d, x, xs, m0, mean, p);
+        else thomas_fwd_chunk<T, false, kTC>(ta, A, k0, N, size2d, x, xs, m0, mean, p);
+        if (k0 + kTC < N) thomas_fwd_chunk<T, false, kTC>(ta, B, k0 + kTC, N, size2d, x, xs, m0, mean, p);
     }
     if (m0) p = 0.0;  // OUT3D(0,0,NZ-2) = 0
     x[(N - 1) * xs] = T(p);
+    if (N < 2) return;
     // back substitution x_kk = y_kk - F_kk x_{kk+1} for kk = N-2 .. 0, walking down in chunks
-#pragma unroll
-    for (int t = 0; t < kTC; t++) nxt[t] = F[max(N - 2 - t, 0) * fs];
+    thomas_tab_down<kTC>(F, fs, N - 2, A);
     k0 = N - 2;
-    for (; k0 - kTC + 1 >= 0; k0 -= kTC) {
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            cur[t] = nxt[t];
-            nxt[t] = F[max(k0 - kTC - t, 0) * fs];
-            const double xm = mean[2 * (k0 - t)], xv = double(x[(k0 - t) * xs]);
-            v[t] = m0 ? xm : xv;
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            p = v[t] - cur[t] * p;
-            v[t] = p;
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) x[(k0 - t) * xs] = T(v[t]);
+    for (; k0 - 2 * kTC + 1 >= 0; k0 -= 2 * kTC) {
+        thomas_tab_down<kTC>(F, fs, k0 - kTC, B);
+        thomas_back_chunk<T, true, kTC>(A, k0, x, xs, m0, mean, p);
+        thomas_tab_down<kTC>(F, fs, k0 - 2 * kTC, A);
+        thomas_back_chunk<T, true, kTC>(B, k0 - kTC, x, xs, m0, mean, p);
     }
-#pragma unroll
-    for (int t = 0; t < kTC; t++) {
-        const int kk = k0 - t;
-        if (kk >= 0) {
-            const double y = m0 ? mean[2 * kk] : double(x[kk * xs]);
-            p = y - nxt[t] * p;
-            x[kk * xs] = T(p);
-        }
+    if (k0 >= 0) {
+        thomas_tab_down<kTC>(F, fs, k0 - kTC, B);
+        if (k0 - kTC + 1 >= 0) thomas_back_chunk<T, true, kTC>(A, k0, x, xs, m0, mean, p);
+        else thomas_back_chunk<T, false, kTC>(A, k0, x, xs, m0, mean, p);
+        if (k0 - kTC >= 0) thomas_back_chunk<T, false, kTC>(B, k0 - kTC, x, xs, m0, mean, p);
     }
 }
 
+// q_kk = x_kk - F_{kk-1} q_{kk-1} for the chunk's levels kk = k0 .. k0+kTC-1 (< N unless FULL), f[t] = F_{k0+t-1}
+template <typename T, bool FULL, int kTC>
+__device__ __forceinline__ void thomas_t_fwd_chunk(const double (&f)[kTC], int k0, int N, T *x, int xs, double &q) {
+    double v[kTC];
+#pragma unroll
+    for (int t = 0; t < kTC; t++) v[t] = double(x[(FULL ? k0 + t : min(k0 + t, N - 1)) * xs]);
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        if (FULL || k0 + t < N) q = v[t] - f[t] * q;
+        v[t] = q;
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++)
+        if (FULL || k0 + t < N) x[(k0 + t) * xs] = T(v[t]);
+}
+// b_kk = z_kk (q_kk - a_{kk+2} b_{kk+1}), stored as b / (ni nj), for kk = k0 .. k0-kTC+1 (>= 0 unless FULL)
+template <typename T, bool FULL, int kTC>
+__device__ __forceinline__ void thomas_t_back_chunk(const ThomasArgs &ta, const double (&z)[kTC], int k0, double size2d, T *x, int xs,
+                                                    double &b) {
+    double v[kTC], ac[kTC];
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        const int kk = FULL ? k0 - t : max(k0 - t, 0);
+        v[t] = double(x[kk * xs]);
+        ac[t] = ta.acoef[kk + 2];
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++) {
+        if (FULL || k0 - t >= 0) b = (v[t] - ac[t] * b) * z[t];
+        v[t] = b / size2d;
+    }
+#pragma unroll
+    for (int t = 0; t < kTC; t++)
+        if (FULL || k0 - t >= 0) x[(k0 - t) * xs] = T(v[t]);
+}
 // Transpose of thomas_part (the adjoint's vertical solve), in place.  The forward map is p = L'^{-1} x / S
 // (L' lower bidiagonal: 1/z_k on the diagonal, a_k below) followed by x_k = p_k - F_k x_{k+1}; for the
 // (0,0) column p_N is replaced by 0.  The transpose runs the two bidiagonal solves transposed and in
 // reverse order: q_1 = x_1, q_k = x_k - F_{k-1} q_{k-1} (q_N = 0 for (0,0)), then b_N = z_N q_N,
 // b_k = z_k (q_k - a_{k+1} b_{k+1}), result b / S.  The singular last pivot of (0,0) is never used.
+// The table entries come in chunks through two register sets that take turns, as in thomas_part.
 template <typename T, int kTC = kThomasChunk>
 __device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs) {
     const int N = ta.nk - 2;
@@ -1000,65 +1063,40 @@ __device__ void thomas_part_t(const ThomasArgs &ta, int kx, int ky, T *x, int xs
     const double *F = ta.ffac + col, *Z = ta.zpiv + col;
     const double size2d = (double)(ta.ni * ta.nj);
     const bool m0 = (kx == 0 && ky == 0);
-    double cur[kTC], nxt[kTC], v[kTC];
+    double A[kTC], B[kTC];
     // q_kk = x_kk - F_{kk-1} q_{kk-1}, kk = 1 .. N-1 (q_0 = x_0)
-#pragma unroll
-    for (int t = 0; t < kTC; t++) nxt[t] = F[min(t, N - 2) * fs];
     double q = double(x[0]);
+    thomas_tab_up<kTC>(F, fs, 0, N - 2, A);
     int k0 = 1;
-    for (; k0 + kTC <= N; k0 += kTC) {
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            cur[t] = nxt[t];
-            nxt[t] = F[min(k0 + kTC + t - 1, N - 2) * fs];
-            v[t] = double(x[(k0 + t) * xs]);
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            q = v[t] - cur[t] * q;
-            v[t] = q;
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) x[(k0 + t) * xs] = T(v[t]);
+    for (; k0 + 2 * kTC <= N; k0 += 2 * kTC) {
+        thomas_tab_up<kTC>(F, fs, k0 + kTC - 1, N - 2, B);
+        thomas_t_fwd_chunk<T, true, kTC>(A, k0, N, x, xs, q);
+        thomas_tab_up<kTC>(F, fs, k0 + 2 * kTC - 1, N - 2, A);
+        thomas_t_fwd_chunk<T, true, kTC>(B, k0 + kTC, N, x, xs, q);
     }
-#pragma unroll
-    for (int t = 0; t < kTC; t++) {
-        const int kk = k0 + t;
-        if (kk < N) {
-            q = double(x[kk * xs]) - nxt[t] * q;
-            x[kk * xs] = T(q);
-        }
+    if (k0 < N) {
+        thomas_tab_up<kTC>(F, fs, k0 + kTC - 1, N - 2, B);
+        if (k0 + kTC <= N) thomas_t_fwd_chunk<T, true, kTC>(A, k0, N, x, xs, q);
+        else thomas_t_fwd_chunk<T, false, kTC>(A, k0, N, x, xs, q);
+        if (k0 + kTC < N) thomas_t_fwd_chunk<T, false, kTC>(B, k0 + kTC, N, x, xs, q);
     }
     // b_{N-1} = z_{N-1} q_{N-1} (0 for (0,0)); b_kk = z_kk (q_kk - a_{kk+2} b_{kk+1}), kk = N-2 .. 0
     double b = m0 ? 0.0 : double(x[(N - 1) * xs]) * Z[(N - 1) * fs];
     x[(N - 1) * xs] = T(b / size2d);
-#pragma unroll
-    for (int t = 0; t < kTC; t++) nxt[t] = Z[max(N - 2 - t, 0) * fs];
+    if (N < 2) return;
+    thomas_tab_down<kTC>(Z, fs, N - 2, A);
     k0 = N - 2;
-    for (; k0 - kTC + 1 >= 0; k0 -= kTC) {
-        double ac[kTC];
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            cur[t] = nxt[t];
-            nxt[t] = Z[max(k0 - kTC - t, 0) * fs];
-            v[t] = double(x[(k0 - t) * xs]);
-            ac[t] = ta.acoef[k0 - t + 2];
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) {
-            b = (v[t] - ac[t] * b) * cur[t];
-            v[t] = b / size2d;
-        }
-#pragma unroll
-        for (int t = 0; t < kTC; t++) x[(k0 - t) * xs] = T(v[t]);
+    for (; k0 - 2 * kTC + 1 >= 0; k0 -= 2 * kTC) {
+        thomas_tab_down<kTC>(Z, fs, k0 - kTC, B);
+        thomas_t_back_chunk<T, true, kTC>(ta, A, k0, size2d, x, xs, b);
+        thomas_tab_down<kTC>(Z, fs, k0 - 2 * kTC, A);
+        thomas_t_back_chunk<T, true, kTC>(ta, B, k0 - kTC, size2d, x, xs, b);
     }
-#pragma unroll
-    for (int t = 0; t < kTC; t++) {
-        const int kk = k0 - t;
-        if (kk >= 0) {
-            b = (double(x[kk * xs]) - ta.acoef[kk + 2] * b) * nxt[t];
-            x[kk * xs] = T(b / size2d);
-        }
+    if (k0 >= 0) {
+        thomas_tab_down<kTC>(Z, fs, k0 - kTC, B);
+        if (k0 - kTC + 1 >= 0) thomas_t_back_chunk<T, true, kTC>(ta, A, k0, size2d, x, xs, b);
+        else thomas_t_back_chunk<T, false, kTC>(ta, A, k0, size2d, x, xs, b);
+        if (k0 - kTC >= 0) thomas_t_back_chunk<T, false, kTC>(ta, B, k0 - kTC, size2d, x, xs, b);
     }
 }
 
