@@ -708,6 +708,19 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
     load_twiddles(twl, tw, n);
     const int jj = j == 0 ? 1 : (j == g.NY - 1 ? g.NY - 2 : j);  // mirrored rows, pressure.cpp:1408-1415
     const double sdt = step * g.dt;
+    // the column coefficients of the block's kc levels (rhou(k), rhow(k+1), rhow(k), mu(k)), read once
+    // into LDS instead of once per element
+    double *colc = (double *)(twl + 2 * n);
+    if (!src) {
+        for (int b = (int)threadIdx.x; b < kc; b += (int)blockDim.x) {
+            const int k = min(kk0 + b, nk2 - 1) + 1;
+            colc[4 * b] = c.rhou[k];
+            colc[4 * b + 1] = c.rhow[k + 1];
+            colc[4 * b + 2] = c.rhow[k];
+            colc[4 * b + 3] = c.mu[k];
+        }
+        __syncthreads();
+    }
     // element idx = i*kc + b.  kU elements per thread per round: every global load of a round is issued
     // (clamped levels, so no load sits behind a branch) before any of them is used; the source-field form
     // and the divergence form are separate loops for the same reason.
@@ -732,10 +745,10 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
                     f3[u] = VP[IX(ia, jj, k)];
                     f4[u] = WP[IX(ia, jj, k + 1)];
                     f5[u] = WP[IX(ia, jj, k)];
-                    ru[u] = c.rhou[k];
-                    rw1[u] = c.rhow[k + 1];
-                    rw0[u] = c.rhow[k];
-                    mk[u] = c.mu[k];
+                    ru[u] = colc[4 * b];
+                    rw1[u] = colc[4 * b + 1];
+                    rw0[u] = colc[4 * b + 2];
+                    mk[u] = colc[4 * b + 3];
                 }
             }
 #pragma unroll
@@ -1554,7 +1567,7 @@ int max_butterflies(const Radix &r, int nbatch, int nthr) {
 }
 
 // LDS of the i passes for kc levels (kc/2 packed pairs) and of the fused j pass, twiddle tables included
-size_t x_pass_lds(int ni, int kc, size_t elt) { return (2 * (size_t)ni * (kc / 2) + 2 * (size_t)ni) * elt; }
+size_t x_pass_lds(int ni, int kc, size_t elt) { return (2 * (size_t)ni * (kc / 2) + 2 * (size_t)ni) * elt + 32 * (size_t)kc; }
 size_t y_pass_lds(int nj, int nk2, size_t elt) {
     return 2 * (size_t)nj * (nk2 + 1) * elt + 8 * (2 * (size_t)nk2 + 1) + 2 * (size_t)nj * elt + 8;
 }
@@ -1570,7 +1583,7 @@ int x_level_chunk(const PressurePlan &pp, const Geo &g, const Radix &rx, size_t 
     int want = fill ? std::max(2, ((pp.nk2 * g.NY / 512) + 1) & ~1) : (pp.nk2 + 1) & ~1;
     if (fill && elt == 8 && want > 16) want = std::max(16, (want / 2) & ~1);
     kc = std::max(2, std::min(kc, want));
-    while (kc > 2 && max_butterflies(rx, kc / 2, kThreadsX) > kMaxBX) kc -= 2;
+    while (kc > 2 && (max_butterflies(rx, kc / 2, kThreadsX) > kMaxBX || x_pass_lds(pp.ni, kc, elt) > kLdsBudget)) kc -= 2;
     return kc;
 }
 
