@@ -39,6 +39,62 @@ __device__ __forceinline__ void cmul(T &ar, T &ai, T br, T bi) {
     ai = i;
 }
 
+// Two or four consecutive elements of a spectrum in one vector access.  A complex value starts at an even
+// element, so a pair is aligned to its size; four elements (the two complex values of a level pair
+// starting at an even level) are aligned to theirs when the level count is even.
+template <typename T>
+struct Vec2T;
+template <>
+struct Vec2T<float> {
+    using type = float2;
+};
+template <>
+struct Vec2T<double> {
+    using type = double2;
+};
+template <typename T>
+__device__ __forceinline__ void ld2(const T *p, T &a, T &b) {
+    typename Vec2T<T>::type v;
+    __builtin_memcpy(&v, __builtin_assume_aligned(p, sizeof(v)), sizeof(v));
+    a = v.x;
+    b = v.y;
+}
+template <typename T>
+__device__ __forceinline__ void st2(T *p, T a, T b) {
+    typename Vec2T<T>::type v;
+    v.x = a;
+    v.y = b;
+    __builtin_memcpy(__builtin_assume_aligned(p, sizeof(v)), &v, sizeof(v));
+}
+template <typename T>
+__device__ __forceinline__ void ld4(const T *p, T (&x)[4]) {
+    if constexpr (sizeof(T) == 4) {
+        float4 v;
+        __builtin_memcpy(&v, __builtin_assume_aligned(p, 16), 16);
+        x[0] = v.x;
+        x[1] = v.y;
+        x[2] = v.z;
+        x[3] = v.w;
+    } else {
+        ld2(p, x[0], x[1]);
+        ld2(p + 2, x[2], x[3]);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void st4(T *p, T a, T b, T c, T d) {
+    if constexpr (sizeof(T) == 4) {
+        float4 v;
+        v.x = a;
+        v.y = b;
+        v.z = c;
+        v.w = d;
+        __builtin_memcpy(__builtin_assume_aligned(p, 16), &v, 16);
+    } else {
+        st2(p, a, b);
+        st2(p + 2, c, d);
+    }
+}
+
 // in-register DFT of length R with exponent sign `sign`
 template <typename T>
 __device__ __forceinline__ void butterfly(int R, T *xr, T *xi, int sign) {
@@ -782,13 +838,17 @@ __device__ __forceinline__ void div_rfft_x_block(const Geo &g, const Cols &c, co
         T zr = re[m * P + pp], zi = im[m * P + pp], wr = re[mm * P + pp], wi = im[mm * P + pp];
         int ka = kk0 + 2 * pp, kb = ka + 1;
         long long o = ((long long)m * g.NY + j) * nk2;
-        if (ka < nk2) {
-            spec[(o + ka) * 2] = T(0.5) * (zr + wr);
-            spec[(o + ka) * 2 + 1] = T(0.5) * (zi - wi);
-        }
-        if (kb < nk2) {
-            spec[(o + kb) * 2] = T(0.5) * (zi + wi);
-            spec[(o + kb) * 2 + 1] = T(0.5) * (wr - zr);
+        if (kb < nk2 && (nk2 & 1) == 0) {  // the pair's four values in one store
+            st4(spec + (o + ka) * 2, T(0.5) * (zr + wr), T(0.5) * (zi - wi), T(0.5) * (zi + wi), T(0.5) * (wr - zr));
+        } else {
+            if (ka < nk2) {
+                spec[(o + ka) * 2] = T(0.5) * (zr + wr);
+                spec[(o + ka) * 2 + 1] = T(0.5) * (zi - wi);
+            }
+            if (kb < nk2) {
+                spec[(o + kb) * 2] = T(0.5) * (zi + wi);
+                spec[(o + kb) * 2 + 1] = T(0.5) * (wr - zr);
+            }
         }
     }
 }
@@ -1126,9 +1186,7 @@ __device__ __forceinline__ void slab_to_lds(const T *__restrict__ spec, long lon
         T vr[kSlabU], vi[kSlabU];
 #pragma unroll
         for (int u = 0; u < kSlabU; u++) {
-            const long long o = base + 2LL * min(i0 + u * nt, n - 1);
-            vr[u] = spec[o];
-            vi[u] = spec[o + 1];
+            ld2(spec + base + 2LL * min(i0 + u * nt, n - 1), vr[u], vi[u]);
         }
 #pragma unroll
         for (int u = 0; u < kSlabU; u++) {
@@ -1166,10 +1224,7 @@ __device__ __forceinline__ void lds_to_slab(T *__restrict__ spec, long long base
 #pragma unroll
         for (int u = 0; u < kSlabU; u++) {
             const int idx = i0 + u * nt;
-            if (idx < n) {
-                spec[base + 2LL * idx] = vr[u];
-                spec[base + 2LL * idx + 1] = vi[u];
-            }
+            if (idx < n) st2(spec + base + 2LL * idx, vr[u], vi[u]);
         }
     }
 }
@@ -1233,17 +1288,18 @@ __device__ __forceinline__ void irfft_x_out_pair_block(const Geo &g, const T *__
         int ms = m <= half ? m : n - m;
         const long long o0 = ((long long)ms * g.NY + jp) * nk2, o1 = ((long long)ms * g.NY + jhalf + jp) * nk2;
         T e[4] = {T(0), T(0), T(0), T(0)}, o[4] = {T(0), T(0), T(0), T(0)};  // a re, a im, b re, b im
-        if (ka < nk2) {
-            e[0] = spec[(o0 + ka) * 2];
-            e[1] = spec[(o0 + ka) * 2 + 1];
-            o[0] = spec[(o1 + ka) * 2];
-            o[1] = spec[(o1 + ka) * 2 + 1];
-        }
-        if (kb < nk2) {
-            e[2] = spec[(o0 + kb) * 2];
-            e[3] = spec[(o0 + kb) * 2 + 1];
-            o[2] = spec[(o1 + kb) * 2];
-            o[3] = spec[(o1 + kb) * 2 + 1];
+        if (kb < nk2 && (nk2 & 1) == 0) {  // the pair's four values in one load
+            ld4(spec + (o0 + ka) * 2, e);
+            ld4(spec + (o1 + ka) * 2, o);
+        } else {
+            if (ka < nk2) {
+                ld2(spec + (o0 + ka) * 2, e[0], e[1]);
+                ld2(spec + (o1 + ka) * 2, o[0], o[1]);
+            }
+            if (kb < nk2) {
+                ld2(spec + (o0 + kb) * 2, e[2], e[3]);
+                ld2(spec + (o1 + kb) * 2, o[2], o[3]);
+            }
         }
 #pragma unroll
         for (int r = 0; r < 2; r++) {
@@ -1330,10 +1386,8 @@ __device__ __forceinline__ void ypass_half_block(const T *__restrict__ in, T *__
         for (int u = 0; u < kU; u++) {
             const int idx = min(idx0 + u * nt, ne - 1), jp = idx / nk2;
             const long long o = base + 2LL * idx;
-            a[u][0] = in[o];
-            a[u][1] = in[o + 1];
-            a[u][2] = in[o + second];
-            a[u][3] = in[o + second + 1];
+            ld2(in + o, a[u][0], a[u][1]);
+            ld2(in + o + second, a[u][2], a[u][3]);
             if (h == 1) {
                 w[u][0] = twy[2 * jp];
                 w[u][1] = twy[2 * jp + 1];
@@ -1386,9 +1440,7 @@ __device__ __forceinline__ void ypass_half_block(const T *__restrict__ in, T *__
             const int idx = idx0 + u * nt;
             if (h == 1) cmul(vr[u], vi[u], T(w[u][0]), -T(w[u][1]));
             if (idx < ne) {
-                const long long o = base + 2LL * ((long long)h * L * nk2 + idx);
-                out[o] = vr[u];
-                out[o + 1] = vi[u];
+                st2(out + base + 2LL * ((long long)h * L * nk2 + idx), vr[u], vi[u]);
             }
         }
     }
