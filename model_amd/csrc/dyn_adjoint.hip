@@ -31,17 +31,9 @@ const int kAdjIds[kAdjFields] = {DYN_U, DYN_V, DYN_W, DYN_TH};
 template <typename T>
 __device__ __forceinline__ T sgn(T x) { return x > T(0) ? T(1) : T(-1); }
 template <typename T>
-__device__ __forceinline__ T i5(T pm2, T pm1, T p0, T p1, T p2, T p3, T sign) {
-    const T c37_60 = T(37.0 / 60.0), c2_15 = T(2.0 / 15.0), c1_60 = T(1.0 / 60.0);
-    T c6 = c37_60 * (p1 + p0) - c2_15 * (p2 + pm1) + c1_60 * (p3 + pm2);
-    T d5 = -(p3 - pm2) + T(5.0) * (p2 - pm1) - T(10.0) * (p1 - p0);
-    return c6 + c1_60 * sign * d5;
-}
+__device__ __forceinline__ T i5(T pm2, T pm1, T p0, T p1, T p2, T p3, T sign) { return face_i5(pm2, pm1, p0, p1, p2, p3, sign); }
 template <typename T>
-__device__ __forceinline__ T i3(T pm1, T p0, T p1, T p2, T sign) {
-    const T c7_12 = T(7.0 / 12.0), c1_12 = T(1.0 / 12.0);
-    return (c7_12 * (p1 + p0) - c1_12 * (p2 + pm1)) + c1_12 * sign * ((p2 - pm1) - T(3.0) * (p1 - p0));
-}
+__device__ __forceinline__ T i3(T pm1, T p0, T p1, T p2, T sign) { return face_i3(pm1, p0, p1, p2, sign); }
 template <typename T>
 __device__ __forceinline__ T i2(T p0, T p1) { return T(0.5) * (p1 + p0); }
 // thread -> point (i, j, k) of the whole array, k fastest (every lane busy whatever NZ is)

@@ -22,6 +22,38 @@ constexpr double kTrigPi = 3.14159265358979324;
 
 void set_error(const std::string &msg);
 
+// Face interpolants of fluxes.cpp:110-206 on explicit operands (p0 = the value on the low side of the
+// face, sign = +-1 from the advecting velocity): the 5th-order upwind-biased one, c6 + sign d5 / 60, and
+// the 3rd-order one.  Every stencil kernel takes them from here, so the fused and per-operator forms of
+// an operator agree to the last bit.  FP64 evaluates the reference's expressions in their order; FP32
+// evaluates the same polynomials with fused multiply-adds (13 and 9 operations instead of 18 and 12,
+// one rounding less per product): the stage kernels are bound by instruction issue.
+template <typename T>
+__device__ __forceinline__ T face_i5(T pm2, T pm1, T p0, T p1, T p2, T p3, T sign) {
+    const T c37_60 = T(37.0 / 60.0), c2_15 = T(2.0 / 15.0), c1_60 = T(1.0 / 60.0);
+    T c6 = c37_60 * (p1 + p0) - c2_15 * (p2 + pm1) + c1_60 * (p3 + pm2);
+    T d5 = -(p3 - pm2) + T(5.0) * (p2 - pm1) - T(10.0) * (p1 - p0);
+    return c6 + c1_60 * sign * d5;
+}
+template <>
+__device__ __forceinline__ float face_i5<float>(float pm2, float pm1, float p0, float p1, float p2, float p3, float sign) {
+    const float c37_60 = float(37.0 / 60.0), c2_15 = float(2.0 / 15.0), c1_60 = float(1.0 / 60.0);
+    const float c6 = __builtin_fmaf(c37_60, p1 + p0, __builtin_fmaf(-c2_15, p2 + pm1, c1_60 * (p3 + pm2)));
+    const float d5 = __builtin_fmaf(5.0f, p2 - pm1, __builtin_fmaf(-10.0f, p1 - p0, pm2 - p3));
+    return __builtin_fmaf(c1_60 * sign, d5, c6);
+}
+template <typename T>
+__device__ __forceinline__ T face_i3(T pm1, T p0, T p1, T p2, T sign) {
+    const T c7_12 = T(7.0 / 12.0), c1_12 = T(1.0 / 12.0);
+    return (c7_12 * (p1 + p0) - c1_12 * (p2 + pm1)) + c1_12 * sign * ((p2 - pm1) - T(3.0) * (p1 - p0));
+}
+template <>
+__device__ __forceinline__ float face_i3<float>(float pm1, float p0, float p1, float p2, float sign) {
+    const float c7_12 = float(7.0 / 12.0), c1_12 = float(1.0 / 12.0);
+    const float c4 = __builtin_fmaf(c7_12, p1 + p0, -(c1_12 * (p2 + pm1)));
+    return __builtin_fmaf(c1_12 * sign, __builtin_fmaf(-3.0f, p1 - p0, p2 - pm1), c4);
+}
+
 #define DYN_HIP(call)                                                                              \
     do {                                                                                           \
         hipError_t _e = (call);                                                                    \

@@ -14,16 +14,10 @@ namespace {
 // ---- face interpolants, fluxes.cpp:110-206 ---------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T interp5(const T *p, long long st, T sign) {
-    const T c37_60 = T(37.0 / 60.0), c2_15 = T(2.0 / 15.0), c1_60 = T(1.0 / 60.0);
-    T c6 = c37_60 * (p[st] + p[0]) - c2_15 * (p[2 * st] + p[-st]) + c1_60 * (p[3 * st] + p[-2 * st]);
-    T d5 = -(p[3 * st] - p[-2 * st]) + T(5.0) * (p[2 * st] - p[-st]) - T(10.0) * (p[st] - p[0]);
-    return c6 + c1_60 * sign * d5;
+    return face_i5(p[-2 * st], p[-st], p[0], p[st], p[2 * st], p[3 * st], sign);
 }
 template <typename T>
-__device__ __forceinline__ T interp3(const T *p, T sign) {
-    const T c7_12 = T(7.0 / 12.0), c1_12 = T(1.0 / 12.0);
-    return (c7_12 * (p[1] + p[0]) - c1_12 * (p[2] + p[-1])) + c1_12 * sign * ((p[2] - p[-1]) - T(3.0) * (p[1] - p[0]));
-}
+__device__ __forceinline__ T interp3(const T *p, T sign) { return face_i3(p[-1], p[0], p[1], p[2], sign); }
 template <typename T>
 __device__ __forceinline__ T interp2(const T *p) { return T(0.5) * (p[1] + p[0]); }
 // signof, fluxes.cpp:295-311 (zero maps to -1)

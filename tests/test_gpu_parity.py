@@ -289,9 +289,9 @@ def test_full_size_fp64_against_oracle(name, steps):
 def test_c2_fp32_one_hour_error_against_fp64():
     """The FP32 form against the FP64 form (which the oracle pins) over the full 1800 steps (1 h) of the
     256x256x64 moist supercell: the relative L2 error of every prognostic and of pi, reported every 300
-    steps.  Moist convection amplifies rounding differences, so the errors grow with time; the bound at
-    one hour is bounded for theta' and w (measured 8.9e-5 and 1.4e-4 on MI355X, against 1e-5 and 7e-6
-    at 300 steps), and every field must stay finite."""
+    steps.  Moist convection amplifies rounding differences, so the errors grow with time (about 2x per
+    300 steps from a few 1e-6 at 300 steps); at one hour theta' and w must be within 1e-4 (measured
+    7.2e-5 and 9.6e-5 on MI355X), and every field must stay finite."""
     c = configs.supercell()
     a = DynCore(c)
     b = DynCore(c, precision=DYN_FP32)
@@ -311,7 +311,7 @@ def test_c2_fp32_one_hour_error_against_fp64():
             row[ids.FIELD_NAMES[f]] = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
         errs[n] = row
         print(f"step {n}: " + " ".join(f"{k} {v:.2e}" for k, v in row.items()), flush=True)
-    assert errs[1800]["th"] < 1e-4 and errs[1800]["w"] < 3e-4
+    assert errs[1800]["th"] < 1e-4 and errs[1800]["w"] < 1e-4
 
 
 def test_c5_full_size_properties():
