@@ -589,6 +589,28 @@ def test_j_pass_forms_against_oracle(jpass, graph):
     assert rel_err(g.get_field(ids.PI, 0), o.get_field(ids.PI, 0)) <= 1e-11
 
 
+@pytest.mark.parametrize("nz", [5, 10, 11, 18, 19, 26])
+@pytest.mark.parametrize("jpass", [1, 2, 3])
+def test_vertical_solve_level_counts_against_oracle(nz, jpass):
+    """The vertical solves walk the levels in chunks of 8, two chunks a loop trip, then what is left: these
+    level counts (nz - 2 = 3, 8, 9, 16, 17, 24) take each path of that walk (no full chunk, one full chunk
+    and nothing after it, a full and a partial one, whole loop trips only, a loop trip and a partial
+    chunk, ...), in the elimination and in the back substitution, in each j-pass form; the FP64 plain
+    form holds the oracle at 1e-11."""
+    from model_amd import core
+    c = configs.small_case(nx=24, ny=32, nz=nz, microphysics=1, amplitude=0.5)
+    o = Oracle(c)
+    g = DynCore(c)
+    g.tune_set(core.DYN_TUNE_PRESSURE_JPASS, jpass)
+    o.step(4)
+    g.step(4)
+    g.sync()
+    for f in (ids.U, ids.V, ids.W, ids.TH, ids.QV, ids.QC, ids.QR):
+        e = rel_err(g.get_field(f, CUR), o.get_field(f, CUR))
+        assert e <= 1e-11, (nz, jpass, ids.FIELD_NAMES[f], e)
+    assert rel_err(g.get_field(ids.PI, 0), o.get_field(ids.PI, 0)) <= 1e-11
+
+
 def test_odd_transform_length_fork_after_i_pass_bitwise():
     """The odd-length solve (full complex spectrum) forks the scalar half after its first pass too: the
     step then gives the bits of the whole stage on one stream, every role of every prognostic."""
