@@ -124,6 +124,32 @@ def test_adjoint_matches_the_oracle_discrete_adjoint(spin_up):
         assert err <= 1e-10, (f, err)
 
 
+@pytest.mark.parametrize("nz", [10, 19])
+def test_adjoint_level_counts_against_the_oracle_discrete_adjoint(nz):
+    """The transposed vertical solves walk the levels in chunks of 8, two chunks a loop trip, then what is
+    left: nz - 2 = 8 takes only the partial chunk after no loop trip, 17 only whole loop trips (12, in the
+    test above, a full and a partial chunk).  From the initial state, 2 steps with a checkpoint every 2,
+    against the FP64 reference discrete adjoint on the CPU."""
+    from oracle.pyoracle import Adjoint
+    c = configs.vortex_in_shear(nx=32, ny=32, nz=nz, steps=8)
+    core = DynCore(c)
+    x0 = _initial(c)
+    Wu, Wv = vorticity_weights(c, radius=400e3, depth=1500.0)
+    set_initial(core, x0)
+    record(core, 2, 2)
+    grads = backward(core, [Wu, Wv, None, None])
+    core.close()
+    ref = Adjoint(c, x0)
+    ref.step(2)
+    want = ref.gradient([Wu, Wv, None, None])
+    ref.close()
+    gmax = max(np.abs(g).max() for g in want)
+    for f, got, exp in zip(FIELDS, grads, want):
+        err = np.abs(got - exp).max() / gmax
+        print(f"nz {nz}: field {f}: max |HIP - oracle adjoint| / max|g| = {err:.2e}")
+        assert err <= 1e-10, (nz, f, err)
+
+
 @pytest.mark.parametrize("nsteps,every", [(1, 1), (6, 4)], ids=["1-step", "6-steps-every-4"])
 def test_adjoint_matches_central_differences(nsteps, every):
     c = _case()
