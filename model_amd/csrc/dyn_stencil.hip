@@ -404,6 +404,24 @@ struct KessTail {
     const double *ray = nullptr;
     __device__ bool in(int i, int j) const { return on && i >= i0 && i < i1 && j >= j0 && j < j1; }
 };
+// x / y in Kessler's FP64 thermodynamics: the division itself in the FP64 form (the reference's
+// arithmetic); in the FP32 form, whose results are rounded to FP32, x times the hardware reciprocal refined
+// by two Newton steps (within an ulp of the FP64 quotient, five dependent operations where the correctly
+// rounded division takes ten; the kernel is bound by instruction issue)
+template <typename T>
+__device__ __forceinline__ double kdiv(double x, double y) {
+    if constexpr (sizeof(T) == 8) {
+        return x / y;
+    } else {
+        double r = __builtin_amdgcn_rcp(y);
+        double e = fma(-y, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-y, r, 1.0);
+        r = fma(r, e, r);
+        return x * r;
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ THP, T *__restrict__ QVP, T *__restrict__ QCP,
                                                  T *__restrict__ QRP, const T *__restrict__ THB, const T *__restrict__ QB,
@@ -425,7 +443,7 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     const R cpRd = R(kCp / kRd);
     T th = THP[p], qv = QVP[p], qc = QCP[p], qr = QRP[p];
     R theta = R(th) + R(THB[p]) + tbk;
-    R pressure = R(PI[p]) / (cp * R(c.tbv[k])) + R(PB[p]);
+    R pressure = kdiv<T>(R(PI[p]), cp * R(c.tbv[k])) + R(PB[p]);
     R vapor = R(qv) + R(QB[p]) + qbk;
     R temperature = theta * pressure;
     R pd = R(kP0) * pow(pressure, cpRd);
@@ -433,11 +451,11 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     T B = qr > T(1.0e-12) ? rhou * T(2.2) * qc * pow(qr, T(0.875)) * dt : T(0);
     T E = T(0);
     if (qr > T(1.0e-12)) {
-        R esl = R(611.2) * exp(R(17.67) * (temperature - R(273.15)) / (temperature - R(29.65)));
-        R qvsat = R(0.62197) * esl / (pd - esl);
+        R esl = R(611.2) * exp(kdiv<T>(R(17.67) * (temperature - R(273.15)), temperature - R(29.65)));
+        R qvsat = kdiv<T>(R(0.62197) * esl, pd - esl);
         T cvent = T(1.6) + T(30.3922) * pow(rhou * qr, T(0.2046));
-        R e = (R(1.0) / R(rhou)) * (((R(1.0) - vapor / qvsat) * R(cvent) * R(pow(rhou * qr, T(0.525)))) /
-                                  (R(2.03e4) + R(9.58e6) / (pd * qvsat))) * R(dt);
+        R e = kdiv<T>(R(1.0), R(rhou)) * kdiv<T>((R(1.0) - kdiv<T>(vapor, qvsat)) * R(cvent) * R(pow(rhou * qr, T(0.525))),
+                                                R(2.03e4) + kdiv<T>(R(9.58e6), pd * qvsat)) * R(dt);
         E = T(e);
         E = fmin(qr, E);
         E = fmax(E, T(0));
@@ -447,30 +465,30 @@ __global__ void __launch_bounds__(256) k_kessler(Geo g, Cols c, T *__restrict__ 
     T total_convert = fmin(A + B, qc);
     qc -= total_convert;
     qr += (total_convert - E);
-    R de = (Lv / (cp * pressure)) * R(E);
+    R de = kdiv<T>(Lv, cp * pressure) * R(E);
     th -= T(de);
     theta = R(th) + R(THB[p]) + tbk;
     vapor = R(qv) + R(QB[p]) + qbk;
     temperature = theta * pressure;
-    R esl = R(611.2) * exp(R(17.67) * (temperature - R(273.15)) / (temperature - R(29.65)));
-    R qvsat = R(0.62197) * esl / (pd - esl);
+    R esl = R(611.2) * exp(kdiv<T>(R(17.67) * (temperature - R(273.15)), temperature - R(29.65)));
+    R qvsat = kdiv<T>(R(0.62197) * esl, pd - esl);
     const R f = R(17.67 * (273.15 - 29.65) * 2.5e6 / 1004.0);
     R tm = temperature - R(29.65);
-    R phi = pd / (pd - esl) * qvsat * f / (tm * tm);
+    R phi = kdiv<T>(kdiv<T>(pd, pd - esl) * qvsat * f, tm * tm);
     R C = R(0);
     // sat: isSaturated of kessler.cpp:129-150, for the turbulence's moist stability (nullptr when off)
     if (vapor > qvsat) {
-        C = (vapor - qvsat) / (R(1.0) + phi);
+        C = kdiv<T>(vapor - qvsat, R(1.0) + phi);
         if (sat) sat[p] = 1;
     } else if (vapor < qvsat && qc > T(0)) {
-        C = -fmin(R(qc), (qvsat - vapor) / (R(1.0) + phi));
+        C = -fmin(R(qc), kdiv<T>(qvsat - vapor, R(1.0) + phi));
         if (sat) sat[p] = -C < R(qc) ? 1 : 0;
     } else if (sat) {
         sat[p] = vapor == qvsat ? 1 : 0;
     }
     qc += T(C);
     qv -= T(C);
-    R dc = (Lv / (cp * pressure)) * C;
+    R dc = kdiv<T>(Lv, cp * pressure) * C;
     th += T(dc);
     if (tail.in(i, j)) {
         auto damp = [&](int kk, T v) { return kk >= tail.lev ? v - T(tail.ray[kk]) * v : v; };
